@@ -1,0 +1,175 @@
+/*
+ * pycompute_amd: C-ABI of the MI355X batched per-voxel Levenberg-Marquardt
+ * fitter (libpycompute_amd.so, built with hipcc for gfx950).
+ *
+ * The reference (turbotage/pycompute) binds its GPU path from Python through
+ * CuPy: the generated CUDA source goes to cp.RawModule (NVRTC) and the kernel
+ * is launched with (grid, block, args) from a Python loop. These entry points
+ * are what that Python layer binds here instead, through ctypes: compile the
+ * generated HIP source with hiprtc, load the code object, launch the fused
+ * kernels on the caller's HIP stream.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative pca_status; the
+ *     message of the last failure on the calling thread is pca_last_error().
+ *   - Pointers are device pointers (hipMalloc / torch tensors) unless stated;
+ *     the library never frees caller memory.
+ *   - Arrays are structure-of-arrays (rows, N), voxel index fastest, with the
+ *     element type the module was generated for (float or double).
+ *   - Launches are asynchronous on `stream` (a hipStream_t, NULL = default).
+ */
+#ifndef PYCOMPUTE_AMD_H
+#define PYCOMPUTE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pca_status {
+  PCA_OK = 0,
+  PCA_ERR_ARG = -1,      /* invalid argument */
+  PCA_ERR_COMPILE = -2,  /* hiprtc compile failed; log in pca_last_error() */
+  PCA_ERR_HIP = -3,      /* HIP runtime error */
+  PCA_ERR_LIMIT = -4     /* size beyond a documented maximum */
+};
+
+/* Maximum number of voxel-uniform consts (nconst * M) passed by value. */
+#define PCA_UC_MAX 256
+
+/* Arguments of one fused LM fit launch (all iterations).
+ * Replaces the reference kernel ABI
+ *   k_second_order_levenberg_marquardt_<n>_<c>_<M>_<t>_<hash>(consts, data,
+ *     lower_bound, upper_bound, tol, mu, eta, acc, dec, params, params_tp,
+ *     step, lam, step_type, f, ftp, g, h, hl, Nprobs)
+ * (pycompute/cuda/lsqnonlin.py:987-1011; emitted
+ *  test/ivim/second_order_levenberg_marquardt_4_1_21_f_f7def86f0a03.cu:519-523),
+ * which the host launched once per iteration (lsqnonlin.py:1058-1073). The
+ * scalars are double here and converted to the kernel's element type inside
+ * the kernel (the reference passed Python floats to float parameters). */
+typedef struct pca_lm_args {
+  const void* data;      /* (M, N) measurements */
+  const void* consts;    /* (nconst*M, N) per-voxel consts; unused by uniform-consts modules */
+  const void* lb;        /* (n, N) lower bounds */
+  const void* ub;        /* (n, N) upper bounds */
+  void* params;          /* (n, N) initial guess in, fitted parameters out */
+  void* lam;             /* (N) damping lambda, in/out (reference setup: 5) */
+  int8_t* step_type;     /* (N) in/out: 0 converged, 1/2 accepted, 4 rejected, |8 non-descent */
+  void* f;               /* (N) in/out: SSR at the start of the last executed iteration */
+  void* first_f;         /* (N) out or NULL: f after the first iteration */
+  void* last_f;          /* (N) out or NULL: f after the last iteration */
+  int32_t* n_iter;       /* (N) incremented by executed iterations, or NULL */
+  void* g_out;           /* (n, N) or NULL: gradient J^T r of the last iteration */
+  void* h_out;           /* (n(n+1)/2, N) or NULL: Hessian, packed lower triangle */
+  void* hl_out;          /* (n(n+1)/2, N) or NULL: damped Hessian */
+  void* step_out;        /* (n, N) or NULL: last solved step */
+  int64_t N;             /* voxels */
+  int32_t iters;         /* iterations (the reference's launches) */
+  int32_t reserved;      /* 0 */
+  double tol, mu, eta, acc, dec;
+} pca_lm_args;
+
+/* Arguments of the standalone SSR kernel: f[v] += sum_i r_i^2, skipping
+ * voxels whose step_type is 0. Replaces F.run(pars, consts, data, step_type, f)
+ * (pycompute/cuda/lsqnonlin.py:137-145, kernel :157-177). */
+typedef struct pca_ssr_args {
+  const void* params;       /* (n, N) */
+  const void* data;         /* (M, N) */
+  const void* consts;       /* (nconst*M, N) per-voxel consts or unused */
+  const int8_t* step_type;  /* (N) or NULL */
+  void* f;                  /* (N) accumulated */
+  int64_t N;
+} pca_ssr_args;
+
+/* Arguments of the model evaluation kernel (model, Jacobian, second
+ * derivatives per measurement). Replaces the per-measurement evaluation of
+ * fghhl (pycompute/cuda/lsqnonlin.py:206-346) as a standalone surface. */
+typedef struct pca_eval_args {
+  const void* params;  /* (n, N) */
+  const void* consts;  /* (nconst*M, N) per-voxel consts or unused */
+  void* m;             /* (M, N) or NULL */
+  void* J;             /* (n*M, N) or NULL, row j*M+i */
+  void* d2;            /* (n(n+1)/2*M, N) or NULL, row l*M+i */
+  int64_t N;
+} pca_eval_args;
+
+typedef struct pca_module_s* pca_module;
+typedef struct pca_kernel_s* pca_kernel;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* pca_last_error(void);
+
+/* Library ABI version (bumped when a struct above changes). */
+int32_t pca_abi_version(void);
+
+/* Compile HIP source with hiprtc into a code object. Replaces
+ * cp.RawModule(code=...) (pycompute/cuda/lsqnonlin.py:1013-1025,
+ * solver.py:584). `options` are hiprtc options (e.g. --offload-arch=gfx950).
+ * On success *code is a malloc'd buffer of *size bytes; release it with
+ * pca_free(). On a compile error the log is in pca_last_error(). Needs no
+ * GPU. */
+int pca_compile(const char* source, const char* program_name, const char* const* options,
+                int32_t n_options, void** code, size_t* size);
+
+/* Free a buffer returned by pca_compile. */
+void pca_free(void* p);
+
+/* Load a code object on the current device. Replaces RawModule loading. */
+int pca_module_load(const void* code, size_t size, pca_module* out);
+int pca_module_unload(pca_module module);
+
+/* Look up a kernel in a module. Replaces mod.get_function(name)
+ * (pycompute/cuda/lsqnonlin.py:1020). */
+int pca_module_kernel(pca_module module, const char* name, pca_kernel* out);
+
+/* Launch the fused LM fit (kernel "pca_lm_fit" of a generated module) over
+ * ceil(N/block) blocks; `block` must equal the block size the module was
+ * generated for. `uconsts` holds the voxel-uniform consts (uconsts_bytes <=
+ * PCA_UC_MAX * element size; NULL for per-voxel modules). Replaces the
+ * per-iteration launch loop of SecondOrderLevenbergMarquardt.run
+ * (pycompute/cuda/lsqnonlin.py:1058-1073). */
+int pca_lm_launch(pca_kernel kernel, const pca_lm_args* args, const void* uconsts,
+                  size_t uconsts_bytes, uint32_t block, void* stream);
+
+/* Launch the standalone SSR kernel ("pca_ssr"). Replaces F.run
+ * (pycompute/cuda/lsqnonlin.py:137-145). */
+int pca_ssr_launch(pca_kernel kernel, const pca_ssr_args* args, const void* uconsts,
+                   size_t uconsts_bytes, uint32_t block, void* stream);
+
+/* Launch the model evaluation kernel ("pca_eval"). */
+int pca_eval_launch(pca_kernel kernel, const pca_eval_args* args, const void* uconsts,
+                    size_t uconsts_bytes, uint32_t block, void* stream);
+
+/* Generic launch of a generated kernel: `params` is the hipModuleLaunchKernel
+ * kernelParams array (one pointer per kernel argument). Replaces
+ * RawKernel.__call__(grid, block, args) for the generated helper kernels. */
+int pca_launch(pca_kernel kernel, uint32_t gx, uint32_t gy, uint32_t gz, uint32_t bx, uint32_t by,
+               uint32_t bz, uint32_t shared_bytes, void* stream, void** params);
+
+/* Batched polynomial least squares: out (deg+1, N) fits y (m, N) on
+ * [1, x, .., x^deg]; x is (m, N) when x_per_voxel, else (m,). deg <= 7.
+ * Replaces bpolyfit (pycompute/cuda/lsq.py:22-30). is_double selects the
+ * element type of x, y and out. */
+int pca_bpolyfit(const void* x, int32_t x_per_voxel, const void* y, int32_t m, int32_t deg,
+                 void* out, int64_t N, int32_t is_double, void* stream);
+
+/* Default initial guess: out (n, N) = clip(nan_to_num(T(P log y)), lb, ub)
+ * with P a device (K, M) double matrix shared by all voxels and T chosen by
+ * kind (0 identity, 1 mono-exponential, 2 stretched exponential, 3 segmented
+ * IVIM). Kind 3 replaces the caller-side init of test/ivim/test_ivim.ipynb
+ * cell 6 (lines 146-165: two bpolyfit calls, exp, f = 1 - S0_hi/S0_lo,
+ * nan_to_num, clip). */
+int pca_loglinear_init(const double* P, int32_t K, int32_t M, const void* y, int32_t kind,
+                       int32_t n, const void* lb, const void* ub, void* out, int64_t N,
+                       int32_t is_double, void* stream);
+
+/* gcnArchName of `device` (e.g. "gfx950:sramecc+:xnack-"). */
+int pca_device_arch(int32_t device, char* buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYCOMPUTE_AMD_H */

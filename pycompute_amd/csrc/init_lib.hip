@@ -1,0 +1,187 @@
+// Initial-guess kernels, compiled into libpycompute_amd.so for gfx950.
+//
+// pca_bpolyfit: batched polynomial least squares of pycompute/cuda/lsq.py:22-30
+//   out[:, v] = argmin_c || A_v c - y_v ||,  A_v = [1, x_v, ..., x_v^deg]
+// The reference forms pinv(A) with a batched SVD; for the full-rank systems it
+// is used on, that equals the least-squares solution computed here from the
+// normal equations in double precision (Cholesky, deg <= 7).
+//
+// pca_loglinear_init: out = T(P log y) per voxel with a voxel-uniform matrix P
+// (K x M, double). With P the pseudo-inverse of a shared design this is the
+// same least-squares fit as bpolyfit on replicated x, and covers the default
+// initial guesses of the built-in models:
+//   kind 0  p = P log y                       (DKI: log-linear OLS)
+//   kind 1  p = [exp(c0), 1/c1]               (T2:  log S = log S0 - TE/T2)
+//   kind 2  p = [exp(c0), c1, 1]              (stretched exp, alpha = 1)
+//   kind 3  segmented IVIM of test/ivim/test_ivim.ipynb cell 6 (:146-165):
+//           rows (c0_lo, c1_lo, c0_hi, c1_hi) of the low-b and high-b fits,
+//           p = [exp(c0_lo), 1 - exp(c0_hi)/exp(c0_lo), c1_lo, c1_hi]
+// followed by nan_to_num (NaN and +-Inf to 0) and clipping to [lb, ub], as the
+// notebook does.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "pycompute_amd.h"
+#include "internal.h"
+
+namespace {
+
+constexpr int kMaxDeg = 7;
+constexpr int kMaxK = 32;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_bpolyfit(const T* __restrict__ x, int x_per_voxel, const T* __restrict__ y, int m, int deg,
+           T* __restrict__ out, long long N) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  const int K = deg + 1;
+  double G[(kMaxDeg + 1) * (kMaxDeg + 1)];
+  double r[kMaxDeg + 1];
+  for (int a = 0; a < K * K; ++a) G[a] = 0.0;
+  for (int a = 0; a < K; ++a) r[a] = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double xi = (double)(x_per_voxel ? x[(long long)i * N + v] : x[i]);
+    const double yi = (double)y[(long long)i * N + v];
+    double pa = 1.0;
+    for (int a = 0; a < K; ++a) {
+      r[a] += pa * yi;
+      double pb = 1.0;
+      for (int b = 0; b <= a; ++b) {
+        G[a * K + b] += pa * pb;
+        pb *= xi;
+      }
+      pa *= xi;
+    }
+  }
+  // Cholesky G = L L^T (lower), then two triangular solves.
+  for (int j = 0; j < K; ++j) {
+    double d = G[j * K + j];
+    for (int q = 0; q < j; ++q) d -= G[j * K + q] * G[j * K + q];
+    d = sqrt(d);
+    G[j * K + j] = d;
+    for (int i = j + 1; i < K; ++i) {
+      double t = G[i * K + j];
+      for (int q = 0; q < j; ++q) t -= G[i * K + q] * G[j * K + q];
+      G[i * K + j] = t / d;
+    }
+  }
+  for (int i = 0; i < K; ++i) {
+    double t = r[i];
+    for (int q = 0; q < i; ++q) t -= G[i * K + q] * r[q];
+    r[i] = t / G[i * K + i];
+  }
+  for (int i = K - 1; i >= 0; --i) {
+    double t = r[i];
+    for (int q = i + 1; q < K; ++q) t -= G[q * K + i] * r[q];
+    r[i] = t / G[i * K + i];
+  }
+  for (int a = 0; a < K; ++a) out[(long long)a * N + v] = (T)r[a];
+}
+
+template <typename T>
+__device__ __forceinline__ T nan_to_num(T a) {
+  return (__builtin_isnan(a) || __builtin_isinf(a)) ? (T)0 : a;
+}
+
+template <typename T>
+__device__ __forceinline__ T t_log(T a);
+template <>
+__device__ __forceinline__ float t_log<float>(float a) { return logf(a); }
+template <>
+__device__ __forceinline__ double t_log<double>(double a) { return log(a); }
+template <typename T>
+__device__ __forceinline__ T t_exp(T a);
+template <>
+__device__ __forceinline__ float t_exp<float>(float a) { return expf(a); }
+template <>
+__device__ __forceinline__ double t_exp<double>(double a) { return exp(a); }
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_loglinear_init(const double* __restrict__ P, int K, int M, const T* __restrict__ y, int kind,
+                 int n, const T* __restrict__ lb, const T* __restrict__ ub, T* __restrict__ out,
+                 long long N) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  double c[kMaxK];
+  for (int k = 0; k < K; ++k) c[k] = 0.0;
+  for (int i = 0; i < M; ++i) {
+    const double ly = (double)t_log(y[(long long)i * N + v]);
+    for (int k = 0; k < K; ++k) c[k] += P[k * M + i] * ly;
+  }
+  T p[kMaxK];
+  if (kind == 0) {
+    for (int k = 0; k < K; ++k) p[k] = (T)c[k];
+  } else if (kind == 1) {
+    p[0] = t_exp((T)c[0]);
+    p[1] = (T)1 / (T)c[1];
+  } else if (kind == 2) {
+    p[0] = t_exp((T)c[0]);
+    p[1] = (T)c[1];
+    p[2] = (T)1;
+  } else {
+    const T s_lo = t_exp((T)c[0]);
+    const T s_hi = t_exp((T)c[2]);
+    p[0] = s_lo;
+    p[1] = (T)1 - s_hi / s_lo;
+    p[2] = (T)c[1];
+    p[3] = (T)c[3];
+  }
+  for (int j = 0; j < n; ++j) {
+    T a = nan_to_num(p[j]);
+    const T lo = lb[(long long)j * N + v], hi = ub[(long long)j * N + v];
+    a = a < lo ? lo : a;
+    a = a > hi ? hi : a;
+    out[(long long)j * N + v] = a;
+  }
+}
+
+}  // namespace
+
+extern "C" int pca_bpolyfit(const void* x, int32_t x_per_voxel, const void* y, int32_t m,
+                            int32_t deg, void* out, int64_t N, int32_t is_double, void* stream) {
+  if (!x || !y || !out || m <= 0 || deg < 0 || deg > kMaxDeg || N < 0)
+    return pca_fail(PCA_ERR_ARG, "pca_bpolyfit: bad argument (deg must be 0..7)");
+  if (N == 0) return PCA_OK;
+  const dim3 grid((unsigned)((N + 255) / 256)), block(256);
+  if (is_double)
+    hipLaunchKernelGGL(k_bpolyfit<double>, grid, block, 0, (hipStream_t)stream,
+                       (const double*)x, x_per_voxel, (const double*)y, m, deg, (double*)out,
+                       (long long)N);
+  else
+    hipLaunchKernelGGL(k_bpolyfit<float>, grid, block, 0, (hipStream_t)stream, (const float*)x,
+                       x_per_voxel, (const float*)y, m, deg, (float*)out, (long long)N);
+  {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PCA_OK : pca_fail(PCA_ERR_HIP, hipGetErrorString(e));
+  }
+}
+
+extern "C" int pca_loglinear_init(const double* P, int32_t K, int32_t M, const void* y,
+                                  int32_t kind, int32_t n, const void* lb, const void* ub,
+                                  void* out, int64_t N, int32_t is_double, void* stream) {
+  if (!P || !y || !lb || !ub || !out || K <= 0 || K > kMaxK || M <= 0 || n <= 0 || n > kMaxK ||
+      kind < 0 || kind > 3 || N < 0)
+    return pca_fail(PCA_ERR_ARG, "pca_loglinear_init: bad argument");
+  if ((kind == 1 && (K != 2 || n != 2)) || (kind == 2 && (K != 2 || n != 3)) ||
+      (kind == 3 && (K != 4 || n != 4)) || (kind == 0 && K != n))
+    return pca_fail(PCA_ERR_ARG, "pca_loglinear_init: K/n do not match the init kind");
+  if (N == 0) return PCA_OK;
+  const dim3 grid((unsigned)((N + 255) / 256)), block(256);
+  if (is_double)
+    hipLaunchKernelGGL(k_loglinear_init<double>, grid, block, 0, (hipStream_t)stream, P, K, M,
+                       (const double*)y, kind, n, (const double*)lb, (const double*)ub,
+                       (double*)out, (long long)N);
+  else
+    hipLaunchKernelGGL(k_loglinear_init<float>, grid, block, 0, (hipStream_t)stream, P, K, M,
+                       (const float*)y, kind, n, (const float*)lb, (const float*)ub,
+                       (float*)out, (long long)N);
+  {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PCA_OK : pca_fail(PCA_ERR_HIP, hipGetErrorString(e));
+  }
+}

@@ -1,0 +1,271 @@
+"""HIP source generation for a model expression.
+
+Replaces the reference's printer and fghhl/f templates
+(pycompute/sym/util.py:32-59, pycompute/cuda/lsqnonlin.py:46-117, 206-346):
+
+* ``res_jac_hes`` builds, for ``model(pars; consts)``, the model value, its
+  gradient and its packed lower-triangle Hessian (l = j(j+1)/2 + k, k <= j),
+  in one hash-consed DAG (the CSE of sym/util.py:100-112).
+* ``emit_model`` prints a HIP ``struct`` with straight-line device code:
+    - ``resid(p, c, y)``          model - y              (the F kernel body)
+    - ``accum(p, c, y, lam, f, g, h, hld)``  one measurement of the fghhl loop:
+        f += r*r; for j, k<=j: jt = J_j J_k, hj = H_jk r + jt, h_l += hj,
+        hld_j += hj + max(lam jt, 1e-4) on the diagonal; g_j += J_j r
+      (lsqnonlin.py:237-256). Structurally zero Hessian entries are dropped at
+      generation time (the reference multiplies a literal 0 by r).
+    - ``fjh(p, c, m, J, d2)``     model, Jacobian, second derivatives
+* Float builds print single-precision literals and functions throughout (the
+  reference leaves decimal literals as double and maps rsqrt to "rsqrtff").
+"""
+from __future__ import annotations
+
+import hashlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import expr as E
+
+_FN_F = {"exp": "PCA_EXPF", "log": "PCA_LOGF", "sqrt": "sqrtf", "sin": "sinf", "cos": "cosf",
+         "tan": "tanf", "sinh": "sinhf", "cosh": "coshf", "tanh": "tanhf", "asin": "asinf",
+         "acos": "acosf", "atan": "atanf", "abs": "fabsf"}
+_FN_D = {"exp": "exp", "log": "log", "sqrt": "sqrt", "sin": "sin", "cos": "cos", "tan": "tan",
+         "sinh": "sinh", "cosh": "cosh", "tanh": "tanh", "asin": "asin", "acos": "acos",
+         "atan": "atan", "abs": "fabs"}
+
+
+def expr_hash(text: str, length: int = 12) -> str:
+    """Kernel-name hash of an expression: md5 hex digest characters [1, 1+length)
+    (pycompute/sym/util.py:61-62)."""
+    return hashlib.md5(text.encode()).hexdigest()[1:length + 1]
+
+
+@dataclass
+class ModelDAG:
+    expr: str
+    pars: list
+    consts: list
+    graph: E.Graph
+    model: E.Node
+    jac: list
+    hes: list                     # packed lower triangle, len n(n+1)/2
+    counts_fjh: dict = field(default_factory=dict)
+    counts_f: dict = field(default_factory=dict)
+
+    @property
+    def nparam(self):
+        return len(self.pars)
+
+    @property
+    def nconst(self):
+        return len(self.consts)
+
+    @property
+    def nhes(self):
+        return len(self.hes)
+
+    def hes_nonzero(self):
+        return [not E.is_zero(h) for h in self.hes]
+
+
+def res_jac_hes(expr: str, pars, consts) -> ModelDAG:
+    """Model, gradient and packed Hessian of ``expr`` w.r.t. ``pars``."""
+    pars, consts = list(pars), list(consts)
+    if len(set(pars) | set(consts)) != len(pars) + len(consts):
+        raise ValueError("parameter and constant names must be distinct")
+    g = E.Graph()
+    m = E.parse(g, expr)
+    unknown = E.free_symbols(m) - set(pars) - set(consts)
+    if unknown:
+        raise ValueError(f"expression uses undeclared symbols: {sorted(unknown)}")
+    memo: dict = {}
+    jac = [E.diff(g, m, p, memo) for p in pars]
+    hes = []
+    for j in range(len(pars)):
+        for k in range(j + 1):
+            hes.append(E.diff(g, jac[j], pars[k], memo))
+    dag = ModelDAG(expr, pars, consts, g, m, jac, hes)
+    dag.counts_fjh = E.op_counts([m] + jac + hes)
+    dag.counts_f = E.op_counts([m])
+    return dag
+
+
+def _lit(v: float, dtype: str) -> str:
+    if dtype == "float32":
+        f = np.float32(v)
+        if not np.isfinite(f):
+            raise ValueError(f"constant {v} is not representable in float32")
+        s = repr(float(f)) if float(f) == int(f) and abs(f) < 1e7 else np.format_float_positional(
+            f, unique=True, trim="0") if 1e-4 <= abs(f) < 1e7 else np.format_float_scientific(
+            f, unique=True, trim="0")
+        if "." not in s and "e" not in s:
+            s += ".0"
+        return s + "f"
+    s = repr(float(v))
+    if "." not in s and "e" not in s and "inf" not in s and "nan" not in s:
+        s += ".0"
+    return s
+
+
+class _Printer:
+    """Straight-line emission of a DAG: one SSA temporary per interior node."""
+
+    def __init__(self, dag: ModelDAG, dtype: str, p_name="p", c_name="c"):
+        self.dag = dag
+        self.dtype = dtype
+        self.fn = _FN_F if dtype == "float32" else _FN_D
+        self.pow = "powf" if dtype == "float32" else "pow"
+        self.sym = {name: f"{p_name}[{k}]" for k, name in enumerate(dag.pars)}
+        self.sym.update({name: f"{c_name}[{k}]" for k, name in enumerate(dag.consts)})
+        self.names: dict = {}
+        self.lines: list = []
+        self.count = 0
+
+    def ref(self, n: E.Node) -> str:
+        if n.op == "num":
+            return _lit(n.value, self.dtype)
+        if n.op == "sym":
+            return self.sym[n.name]
+        return self.names[n.id]
+
+    def _tmp(self, text: str) -> str:
+        name = f"t{self.count}"
+        self.count += 1
+        self.lines.append(f"const real {name} = {text};")
+        return name
+
+    def emit(self, outputs):
+        for n in E.topo(outputs):
+            if n.op in ("num", "sym") or n.id in self.names:
+                continue
+            a = [self.ref(x) for x in n.args]
+            if n.op == "add":
+                txt = f"{a[0]} + {a[1]}"
+            elif n.op == "neg":
+                txt = f"-{a[0]}"
+            elif n.op == "mul":
+                txt = f"{a[0]}*{a[1]}"
+            elif n.op == "div":
+                txt = f"{a[0]}/{a[1]}"
+            elif n.op == "pow":
+                e = E.int_power(n)
+                if e is not None:
+                    txt = self._int_pow(a[0], abs(e))
+                    if e < 0:
+                        txt = f"{_lit(1.0, self.dtype)}/({txt})"
+                elif n.args[1].op == "num" and n.args[1].value == 0.5:
+                    txt = f"{self.fn['sqrt']}({a[0]})"
+                else:
+                    txt = f"{self.pow}({a[0]}, {a[1]})"
+            elif n.op == "fn":
+                txt = f"{self.fn[n.name]}({a[0]})"
+            else:  # pragma: no cover
+                raise ValueError(n.op)
+            self.names[n.id] = self._tmp(txt)
+
+    def _int_pow(self, x: str, k: int) -> str:
+        # exponentiation by squaring on SSA temporaries
+        result, base = None, x
+        while True:
+            if k & 1:
+                result = base if result is None else self._tmp(f"{result}*{base}")
+            k >>= 1
+            if not k:
+                break
+            base = self._tmp(f"{base}*{base}")
+        return result
+
+
+def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32") -> str:
+    """HIP source of the model struct (see module docstring)."""
+    n, nh = dag.nparam, dag.nhes
+    floor = _lit(1e-4, dtype)
+    out = [f"struct {struct_name} {{",
+           f"  static constexpr int NP = {n};",
+           f"  static constexpr int NC = {dag.nconst};",
+           f"  static constexpr int NH = {nh};"]
+
+    # resid: model - y (the F kernel body, lsqnonlin.py:60-66)
+    pr = _Printer(dag, dtype)
+    pr.emit([dag.model])
+    body = pr.lines + [f"return {pr.ref(dag.model)} - y;"]
+    out.append("  __device__ static __forceinline__ real resid(const real* __restrict__ p, "
+               "const real* __restrict__ c, real y) {")
+    out += ["    " + s for s in body]
+    out.append("  }")
+
+    # accum: one measurement of fghhl (lsqnonlin.py:214-258)
+    pr = _Printer(dag, dtype)
+    nz = dag.hes_nonzero()
+    pr.emit([dag.model] + dag.jac + [h for h, z in zip(dag.hes, nz) if z])
+    body = list(pr.lines)
+    body.append(f"const real r = {pr.ref(dag.model)} - y;")
+    body.append("f += r*r;")
+    l = 0
+    for j in range(n):
+        Jj = pr.ref(dag.jac[j])
+        for k in range(j + 1):
+            Jk = pr.ref(dag.jac[k])
+            if nz[l]:
+                hj = f"{pr.ref(dag.hes[l])}*r + jt{l}"
+            else:
+                hj = f"jt{l}"
+            body.append(f"{{ const real jt{l} = {Jj}*{Jk}; const real hj = {hj}; h[{l}] += hj;"
+                        + (f" hld[{j}] += hj + pca_fmax(lam*jt{l}, {floor});" if j == k else "")
+                        + " }")
+            l += 1
+        body.append(f"g[{j}] += {Jj}*r;")
+    out.append("  __device__ static __forceinline__ void accum(const real* __restrict__ p, "
+               "const real* __restrict__ c, real y, real lam, real& f, real* g, real* h, "
+               "real* hld) {")
+    out += ["    " + s for s in body]
+    out.append("  }")
+
+    # fjh: model, Jacobian, second derivatives (for the eval kernels)
+    pr = _Printer(dag, dtype)
+    pr.emit([dag.model] + dag.jac + dag.hes)
+    body = list(pr.lines)
+    body.append(f"m = {pr.ref(dag.model)};")
+    body += [f"J[{j}] = {pr.ref(dag.jac[j])};" for j in range(n)]
+    body += [f"d2[{l}] = {pr.ref(dag.hes[l])};" for l in range(nh)]
+    out.append("  __device__ static __forceinline__ void fjh(const real* __restrict__ p, "
+               "const real* __restrict__ c, real& m, real* J, real* d2) {")
+    out += ["    " + s for s in body]
+    out.append("  }")
+    out.append("};")
+    return "\n".join(out) + "\n"
+
+
+def flops_per_iteration(dag: ModelDAG, M: int) -> dict:
+    """Algorithmic FLOPs of one active LM iteration (exp counted as 1 FLOP and
+    also reported separately).
+
+    F_it = M (E_fjh + 1 + 2 + 2n + 2 nh + 2 nnz + 2n) + M (E_f + 3)
+           + F_solve(n) + F_step(n)
+    with E_fjh/E_f the op counts of the CSE'd DAGs (transcendentals included
+    as one each), nnz the structurally non-zero Hessian entries, and
+      F_solve(n) = (n sqrt + n div + 2 nh) + sum_j [j + 2j(n-1-j) + 2 + 3(n-1-j)] + 4
+                   + 3 n(n-1)/2 + n + n(n-1) + n
+      F_step(n)  = (4n + 2) + n + (3 nh + 2n + 4).
+    The converging iteration stops after the convergence test:
+      F_conv = M (E_fjh + 1 + 2 + 2n + 2 nh + 2 nnz + 2n) + F_solve(n) + 4n + 2.
+    """
+    n, nh = dag.nparam, dag.nhes
+    nnz = sum(dag.hes_nonzero())
+
+    def tot(c):
+        return c["flops"] + c["exp"] + c["log"] + c["pow"] + c["sqrt"] + c["other_fn"]
+
+    e_fjh, e_f = tot(dag.counts_fjh), tot(dag.counts_f)
+    gmw = sum(j + 2 * j * (n - 1 - j) + 2 + 3 * (n - 1 - j) for j in range(n)) + 4
+    f_solve = (2 * n + 2 * nh) + gmw + 3 * n * (n - 1) // 2 + n + n * (n - 1) + n
+    f_step = (4 * n + 2) + n + (3 * nh + 2 * n + 4)
+    fghhl = M * (e_fjh + 1 + 2 + 2 * n + 2 * nh + 2 * nnz + 2 * n)
+    trial = M * (e_f + 3)
+    return {
+        "F_it": fghhl + trial + f_solve + f_step,
+        "F_conv": fghhl + f_solve + 4 * n + 2,
+        "exp_it": M * (dag.counts_fjh["exp"] + dag.counts_f["exp"]),
+        "exp_conv": M * dag.counts_fjh["exp"],
+        "E_fjh": e_fjh, "E_f": e_f, "nnz": nnz, "F_solve": f_solve, "F_step": f_step,
+    }
