@@ -340,6 +340,15 @@ void gmw81_solver(int n, T* mat, const T* rhs, T* sol) {
 // One voxel's LM state and iteration (.cu:416-517).
 // ---------------------------------------------------------------------------
 
+// A generated model (the project's code generator compiled for the host; see
+// oracle/genmodel.py): one measurement of the fghhl accumulation and the
+// residual, with the same straight-line code the GPU runs. Used by the strict
+// build's bit-exact replay; the built-in models above are independent of it.
+template <typename T>
+using GenAccum = void (*)(const T* p, const T* c, T y, T lam, T* f, T* g, T* h, T* hld);
+template <typename T>
+using GenResid = T (*)(const T* p, const T* c, T y);
+
 template <typename T>
 struct Problem {
   int model, n, nc, M;
@@ -350,6 +359,8 @@ struct Problem {
   const T* lb;
   const T* ub;
   T tol, mu, eta, acc, dec;
+  GenAccum<T> gen_accum = nullptr;
+  GenResid<T> gen_resid = nullptr;
 };
 
 template <typename T>
@@ -363,10 +374,11 @@ inline void load_consts(const Problem<T>& P, long v, int i, T* c) {
 template <typename T>
 T ssr(const Problem<T>& P, long v, const T* p) {
   T f = T(0);
-  T c[4];
+  T c[8];
   for (int i = 0; i < P.M; ++i) {
     load_consts(P, v, i, c);
-    T res = model_f(P.model, p, c) - P.data[(long)i * P.N + v];
+    T res = P.gen_resid ? P.gen_resid(p, c, P.data[(long)i * P.N + v])
+                        : model_f(P.model, p, c) - P.data[(long)i * P.N + v];
     f += res * res;
   }
   return f;
@@ -381,13 +393,27 @@ bool lm_iteration(const Problem<T>& P, long v, T* p, T& lam, int8_t& st, T& f) {
   const int n = P.n;
   const int nh = n * (n + 1) / 2;
   T g[kMaxNP], h[kMaxNH], hl[kMaxNH], J[kMaxNP], d2[kMaxNH], hes[kMaxNH];
-  T c[4];
+  T c[8];
   f = T(0);
   for (int j = 0; j < n; ++j) g[j] = T(0);
   for (int l = 0; l < nh; ++l) { h[l] = T(0); hl[l] = T(0); }
 
+  if (P.gen_accum) {
+    // generated accumulation: h (undamped, off-diagonal equal to hl) and the
+    // damped diagonal hld
+    T hld[kMaxNP];
+    for (int j = 0; j < n; ++j) hld[j] = T(0);
+    for (int i = 0; i < P.M; ++i) {
+      load_consts(P, v, i, c);
+      P.gen_accum(p, c, P.data[(long)i * P.N + v], lam, &f, g, h, hld);
+    }
+    int l = 0;
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k <= j; ++k, ++l) hl[l] = (j == k) ? hld[j] : h[l];
+  }
+
   // fghhl (.cu:17-72)
-  for (int i = 0; i < P.M; ++i) {
+  for (int i = 0; i < (P.gen_accum ? 0 : P.M); ++i) {
     load_consts(P, v, i, c);
     T m = T(0);
     model_fjh(P.model, p, c, m, J, d2);
@@ -505,7 +531,8 @@ bool lm_iteration(const Problem<T>& P, long v, T* p, T& lam, int8_t& st, T& f) {
 template <typename T>
 int lm_run(const Problem<T>& P, T* params, T* lam, int8_t* step_type, T* f, T* first_f, T* last_f,
            int32_t* n_iter, int iters, T* trace_f, T* trace_lam, int8_t* trace_st, int nthreads) {
-  if (P.n != model_nparam(P.model) || P.nc != model_nconst(P.model)) return -1;
+  if (!P.gen_accum && (P.n != model_nparam(P.model) || P.nc != model_nconst(P.model))) return -1;
+  if (P.n < 1 || P.n > kMaxNP || P.nc < 1 || P.nc > 8) return -1;
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 64)
   for (long v = 0; v < P.N; ++v) {
@@ -534,7 +561,8 @@ int lm_run(const Problem<T>& P, T* params, T* lam, int8_t* step_type, T* f, T* f
 
 template <typename T>
 int ssr_run(const Problem<T>& P, const T* params, T* out, int nthreads) {
-  if (P.n != model_nparam(P.model) || P.nc != model_nconst(P.model)) return -1;
+  if (!P.gen_resid && (P.n != model_nparam(P.model) || P.nc != model_nconst(P.model))) return -1;
+  if (P.n < 1 || P.n > kMaxNP || P.nc < 1 || P.nc > 8) return -1;
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(static)
   for (long v = 0; v < P.N; ++v) {
@@ -594,6 +622,25 @@ int model_eval(int model, int M, long N, const T* consts, int cpv, const T* para
                                   int nthreads) {                                                \
     Problem<T> P{model, n, nc, M, N, consts, consts_per_voxel, data, nullptr, nullptr,           \
                  T(0), T(0), T(0), T(0), T(0)};                                                  \
+    return ssr_run(P, params, out, nthreads);                                                    \
+  }                                                                                              \
+  extern "C" int oracle_lm_run_gen_##SUF(                                                        \
+      void* accum, void* resid, int n, int nc, int M, long N, const T* consts,                   \
+      int consts_per_voxel, const T* data, const T* lb, const T* ub, T* params, T* lam,          \
+      int8_t* step_type, T* f, T* first_f, T* last_f, int32_t* n_iter, int iters, T tol, T mu,   \
+      T eta, T acc, T dec, int nthreads) {                                                       \
+    Problem<T> P{-1, n, nc, M, N, consts, consts_per_voxel, data, lb, ub, tol, mu, eta, acc,     \
+                 dec, (GenAccum<T>)accum, (GenResid<T>)resid};                                   \
+    if (!accum || !resid) return -2;                                                             \
+    return lm_run(P, params, lam, step_type, f, first_f, last_f, n_iter, iters, (T*)nullptr,     \
+                  (T*)nullptr, (int8_t*)nullptr, nthreads);                                      \
+  }                                                                                              \
+  extern "C" int oracle_ssr_gen_##SUF(void* resid, int n, int nc, int M, long N,                 \
+                                      const T* consts, int consts_per_voxel, const T* data,      \
+                                      const T* params, T* out, int nthreads) {                   \
+    Problem<T> P{-1, n, nc, M, N, consts, consts_per_voxel, data, nullptr, nullptr,              \
+                 T(0), T(0), T(0), T(0), T(0), nullptr, (GenResid<T>)resid};                     \
+    if (!resid) return -2;                                                                       \
     return ssr_run(P, params, out, nthreads);                                                    \
   }                                                                                              \
   extern "C" int oracle_model_eval_##SUF(int model, int M, long N, const T* consts, int cpv,      \

@@ -18,13 +18,32 @@ typedef double real;
 #define PCA_UC_MAX 256
 #endif
 
-#ifdef PCA_FAST_EXP
-// v_exp_f32 on x*log2(e): relative error about |x| * 6e-8.
-#define PCA_EXPF(x) __expf(x)
-#define PCA_LOGF(x) __logf(x)
+// Transcendentals used by the generated model code. The strict build uses the
+// reproducible implementations of strict_math.h (concatenated before this
+// file) and is compiled with -ffp-contract=off, so the CPU oracle can replay
+// it bit for bit. The fast-exp build uses v_exp_f32 on x*log2(e) for float
+// (relative error about |x| * 6e-8).
+__device__ __forceinline__ float pca_exp(float x) { return expf(x); }
+__device__ __forceinline__ double pca_exp(double x) { return exp(x); }
+__device__ __forceinline__ float pca_exp_fast(float x) { return __expf(x); }
+__device__ __forceinline__ double pca_exp_fast(double x) { return exp(x); }
+__device__ __forceinline__ float pca_log(float x) { return logf(x); }
+__device__ __forceinline__ double pca_log(double x) { return log(x); }
+__device__ __forceinline__ float pca_pow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double pca_pow(double x, double y) { return pow(x, y); }
+
+#if defined(PCA_STRICT)
+#define PCA_EXP(x) pca_exp_strict(x)
+#define PCA_LOG(x) pca_log_strict(x)
+#define PCA_POW(x, y) pca_pow_strict(x, y)
+#elif defined(PCA_FAST_EXP)
+#define PCA_EXP(x) pca_exp_fast(x)
+#define PCA_LOG(x) pca_log(x)
+#define PCA_POW(x, y) pca_pow(x, y)
 #else
-#define PCA_EXPF(x) expf(x)
-#define PCA_LOGF(x) logf(x)
+#define PCA_EXP(x) pca_exp(x)
+#define PCA_LOG(x) pca_log(x)
+#define PCA_POW(x, y) pca_pow(x, y)
 #endif
 
 __device__ __forceinline__ float pca_fmax(float a, float b) { return fmaxf(a, b); }

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
-from .program import Module, read_csrc
+from .program import BASE_OPTIONS, Module, read_csrc
 from .sym import hipgen
 
 DEFAULT_BLOCK = 256
@@ -44,6 +44,7 @@ class LMSpec:
     block: int = DEFAULT_BLOCK
     consts_per_voxel: bool = False
     fast_exp: bool = False
+    strict: bool = False
 
     @property
     def nparam(self):
@@ -75,7 +76,11 @@ class LMProgram:
         self.dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
         self.funcid = spec.funcid()
         self.source = build_source(spec, self.dag)
-        self.module = Module(self.source, self.funcid)
+        options = list(BASE_OPTIONS)
+        if spec.strict:
+            options.append("-ffp-contract=off")
+        self.module = Module(self.source, self.funcid + ("_strict" if spec.strict else ""),
+                             options)
         self.flops = hipgen.flops_per_iteration(self.dag, spec.M)
 
     @property
@@ -105,8 +110,11 @@ def build_source(spec: LMSpec, dag) -> str:
         head.append("#define PCA_CONSTS_PER_VOXEL 1")
     if spec.fast_exp:
         head.append("#define PCA_FAST_EXP 1")
+    if spec.strict:
+        head.append("#define PCA_STRICT 1")
     return "\n".join([
         "\n".join(head),
+        read_csrc("strict_math.h"),
         read_csrc("prelude.hip"),
         hipgen.emit_model(dag, sname, spec.dtype),
         f"typedef {sname} Model;",

@@ -137,7 +137,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
     """Batched second-order LM with the GMW81 step (lsqnonlin.py:949-1073)."""
 
     def __init__(self, *args, block: int = lm.DEFAULT_BLOCK, fast_exp: bool = False,
-                 keep_workspace: bool | None = None, **kwargs):
+                 keep_workspace: bool | None = None, strict: bool = False, **kwargs):
         if (args and isinstance(args[0], str)) or "expr" in kwargs:
             # notebook form: (expr, pars_str, consts_str, ndata, dtype, write_to_file)
             a = _bind(args, kwargs, ("expr", "pars_str", "consts_str", "ndata", "dtype",
@@ -176,11 +176,14 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         self.consts_str = list(model_expr.consts_str) if self.expr_true else None
         self.block = int(block)
         self.fast_exp = bool(fast_exp)
+        # strict: reproducible transcendentals and no FMA contraction, so the
+        # CPU oracle replays the fit bit for bit
+        self.strict = bool(strict)
         self.keep_workspace = (self.nparam <= 8) if keep_workspace is None else bool(keep_workspace)
         if self.expr_true:
             self._spec_uniform = lm.LMSpec(self.expr, tuple(self.pars_str), tuple(self.consts_str),
                                            self.ndata, self.dtype_name, self.block, False,
-                                           self.fast_exp)
+                                           self.fast_exp, self.strict)
             self.funcid = self._spec_uniform.funcid()
         else:
             self.funcid = self._plugin.funcid
@@ -212,7 +215,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             self.code = self.program.source
         else:
             spec = lm.LMSpec(self.expr, tuple(self.pars_str), tuple(self.consts_str), self.ndata,
-                             self.dtype_name, self.block, consts_per_voxel, self.fast_exp)
+                             self.dtype_name, self.block, consts_per_voxel, self.fast_exp,
+                             self.strict)
             try:
                 self.program = lm.get_program(spec)
             except _lib.PcaError:

@@ -21,6 +21,7 @@ CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
 ARCH = "gfx950"
 # Packed-FP32 VALU forms are kept out of every kernel compiled here.
 BASE_OPTIONS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17",
+                "-fhip-fp32-correctly-rounded-divide-sqrt",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 

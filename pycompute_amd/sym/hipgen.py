@@ -26,10 +26,10 @@ import numpy as np
 
 from . import expr as E
 
-_FN_F = {"exp": "PCA_EXPF", "log": "PCA_LOGF", "sqrt": "sqrtf", "sin": "sinf", "cos": "cosf",
+_FN_F = {"exp": "PCA_EXP", "log": "PCA_LOG", "sqrt": "sqrtf", "sin": "sinf", "cos": "cosf",
          "tan": "tanf", "sinh": "sinhf", "cosh": "coshf", "tanh": "tanhf", "asin": "asinf",
          "acos": "acosf", "atan": "atanf", "abs": "fabsf"}
-_FN_D = {"exp": "exp", "log": "log", "sqrt": "sqrt", "sin": "sin", "cos": "cos", "tan": "tan",
+_FN_D = {"exp": "PCA_EXP", "log": "PCA_LOG", "sqrt": "sqrt", "sin": "sin", "cos": "cos", "tan": "tan",
          "sinh": "sinh", "cosh": "cosh", "tanh": "tanh", "asin": "asin", "acos": "acos",
          "atan": "atan", "abs": "fabs"}
 
@@ -114,7 +114,7 @@ class _Printer:
         self.dag = dag
         self.dtype = dtype
         self.fn = _FN_F if dtype == "float32" else _FN_D
-        self.pow = "powf" if dtype == "float32" else "pow"
+        self.pow = "PCA_POW"
         self.sym = {name: f"{p_name}[{k}]" for k, name in enumerate(dag.pars)}
         self.sym.update({name: f"{c_name}[{k}]" for k, name in enumerate(dag.consts)})
         self.names: dict = {}
