@@ -1,0 +1,66 @@
+"""The C-ABI library loads, exports every entry point include/pycompute_amd.h
+declares, compiles with hiprtc, and rejects bad arguments -- all without a
+GPU (no kernel is launched here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from pycompute_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "pycompute_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(pca_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("pca_compile", "pca_module_load", "pca_lm_launch", "pca_ssr_launch",
+              "pca_eval_launch", "pca_bpolyfit", "pca_loglinear_init", "pca_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.pca_abi_version() == _lib.ABI_VERSION
+
+
+def test_struct_layouts_match_header():
+    # pca_lm_args: 15 pointers, int64 N, 2 x int32, 5 doubles
+    assert ctypes.sizeof(_lib.LMArgs) == 15 * 8 + 8 + 8 + 5 * 8
+    assert _lib.LMArgs.tol.offset == 136
+    assert ctypes.sizeof(_lib.SSRArgs) == 6 * 8
+    assert ctypes.sizeof(_lib.EvalArgs) == 6 * 8
+
+
+def test_hiprtc_compile_ok_and_error():
+    code = _lib.compile_source('extern "C" __global__ void k(float* x) { x[threadIdx.x] = 1.0f; }',
+                               "probe.hip", ["--offload-arch=gfx950", "-O3"])
+    assert code[:4] == b"\x7fELF"
+    with pytest.raises(_lib.PcaError) as e:
+        _lib.compile_source("this is not C++", "bad.hip", ["--offload-arch=gfx950"])
+    assert "hiprtc" in str(e.value)
+
+
+def test_argument_errors_without_gpu():
+    L = _lib.lib()
+    a = _lib.LMArgs()
+    rc = L.pca_lm_launch(None, ctypes.byref(a), None, 0, 256, None)
+    assert rc == -1 and b"NULL" in L.pca_last_error()
+    L.pca_bpolyfit.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                               ctypes.c_void_p]
+    assert L.pca_bpolyfit(1, 0, 1, 4, 9, 1, 10, 0, None) == -1
+    assert b"deg" in L.pca_last_error()
+    big = (ctypes.c_char * (_lib.PCA_UC_MAX * 8 + 8))()
+    a.data = a.lb = a.ub = a.params = a.lam = a.step_type = a.f = 1
+    rc = L.pca_lm_launch(None, ctypes.byref(a), big, ctypes.sizeof(big), 256, None)
+    assert rc == -4 and b"PCA_UC_MAX" in L.pca_last_error()

@@ -1,0 +1,180 @@
+"""Code generator checks (no GPU): derivatives against sympy and finite
+differences, naming against the reference's emitted kernel, op counts, and
+hiprtc compile-only builds for gfx950."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from pycompute_amd.models import IVIM, MODELS
+from pycompute_amd.sym import expr as E
+from pycompute_amd.sym import hipgen
+
+USER_EXPRS = [
+    ("a*exp(-x/b) + c", ["a", "b", "c"], ["x"]),
+    ("a*sin(w*t + ph)**2 + log(1 + b*t)", ["a", "w", "ph", "b"], ["t"]),
+    ("sqrt(a*x + b)/(1 + c*x**3) - a**2", ["a", "b", "c"], ["x"]),
+    ("S0*exp(-(b*D)**alpha)", ["S0", "D", "alpha"], ["b"]),
+    ("p0*tanh(p1*u)^2 - p2/(u + p1)", ["p0", "p1", "p2"], ["u"]),
+]
+
+
+def _eval_dag(dag, P, C):
+    env = dict(zip(dag.pars, P))
+    env.update(zip(dag.consts, C))
+    memo = {}
+    m = E.evaluate(dag.model, env, memo)
+    J = [E.evaluate(j, env, memo) for j in dag.jac]
+    H = [E.evaluate(h, env, memo) for h in dag.hes]
+    return m, J, H
+
+
+def _points(dag, rng, n=64):
+    P = [rng.uniform(0.3, 1.5, n) for _ in dag.pars]
+    C = [rng.uniform(0.2, 2.0, n) for _ in dag.consts]
+    return P, C
+
+
+@pytest.mark.parametrize("expr,pars,consts", USER_EXPRS + [(IVIM.expr, IVIM.pars, IVIM.consts)])
+def test_dag_matches_sympy(expr, pars, consts):
+    import sympy as sp
+    dag = hipgen.res_jac_hes(expr, pars, consts)
+    syms = {s: sp.Symbol(s) for s in list(pars) + list(consts)}
+    se = sp.sympify(expr.replace("^", "**"), locals=syms)
+    P_, C_ = [syms[p] for p in pars], [syms[c] for c in consts]
+    rng = np.random.default_rng(1)
+    P, C = _points(dag, rng)
+    m, J, H = _eval_dag(dag, P, C)
+    args = P + C
+    np.testing.assert_allclose(m, sp.lambdify(P_ + C_, se)(*args), rtol=1e-12)
+    l = 0
+    for j, pj in enumerate(P_):
+        dj = sp.diff(se, pj)
+        np.testing.assert_allclose(np.broadcast_to(J[j], (64,)),
+                                   np.broadcast_to(sp.lambdify(P_ + C_, dj)(*args), (64,)),
+                                   rtol=1e-10, atol=1e-12)
+        for k in range(j + 1):
+            dk = sp.diff(dj, P_[k])
+            np.testing.assert_allclose(np.broadcast_to(H[l], (64,)),
+                                       np.broadcast_to(sp.lambdify(P_ + C_, dk)(*args), (64,)),
+                                       rtol=1e-9, atol=1e-11)
+            l += 1
+
+
+@pytest.mark.parametrize("name", ["ivim", "t2", "se", "dki"])
+def test_builtin_models_finite_differences(name):
+    m = MODELS[name]
+    dag = hipgen.res_jac_hes(m.expr, m.pars, m.consts)
+    rng = np.random.default_rng(2)
+    from pycompute_amd import synth
+    w = {"ivim": lambda: synth.ivim_c1(8, 1), "t2": lambda: synth.t2(8, 3),
+         "se": lambda: synth.se(8, 4), "dki": lambda: synth.dki(8, 6)}[name]()
+    P = [w["truth"][j] for j in range(len(m.pars))]
+    for i in rng.choice(w["consts"].shape[1], 4, replace=False):
+        C = [np.full(8, w["consts"][k][i]) for k in range(len(m.consts))]
+        _, J, H = _eval_dag(dag, P, C)
+        # complex-step differentiation: d f/dx = Im f(x + i h) / h, free of
+        # cancellation, so the comparison can be tight
+        h = 1e-30
+        l = 0
+        for j in range(len(P)):
+            Pc = [p.astype(complex) for p in P]
+            Pc[j] = Pc[j] + 1j * h
+            mc, Jc, _ = _eval_dag(dag, Pc, C)
+            np.testing.assert_allclose(np.broadcast_to(J[j], (8,)),
+                                       np.broadcast_to(np.imag(mc) / h, (8,)), rtol=1e-11,
+                                       atol=1e-300)
+            # H is packed lower-triangular: entry (j, k), k <= j, is d J_j / d p_k
+            for k in range(j + 1):
+                Pk = [p.astype(complex) for p in P]
+                Pk[k] = Pk[k] + 1j * h
+                _, Jk, _ = _eval_dag(dag, Pk, C)
+                np.testing.assert_allclose(np.broadcast_to(H[l], (8,)),
+                                           np.broadcast_to(np.imag(Jk[j]) / h, (8,)),
+                                           rtol=1e-10, atol=1e-300)
+                l += 1
+
+
+def test_reference_kernel_name():
+    """test/ivim/second_order_levenberg_marquardt_4_1_21_f_f7def86f0a03.cu is the
+    reference's emitted name for the IVIM model with 21 measurements."""
+    from pycompute_amd.lm import LMSpec
+    assert hipgen.expr_hash(IVIM.expr, 12) == "f7def86f0a03"
+    spec = LMSpec(IVIM.expr, IVIM.pars, IVIM.consts, 21)
+    assert spec.funcid() == "second_order_levenberg_marquardt_4_1_21_f_f7def86f0a03"
+
+
+def test_ivim_structure_and_counts():
+    dag = hipgen.res_jac_hes(IVIM.expr, IVIM.pars, IVIM.consts)
+    # packed Hessian zeros of IVIM: (S0,S0), (f,f), (D_1,D_2) -- as in the
+    # reference's emitted hes[0], hes[2], hes[8] = 0.0f*res
+    assert dag.hes_nonzero() == [False, True, False, True, True, True, True, True, False, True]
+    assert dag.counts_fjh["exp"] == 2 and dag.counts_f["exp"] == 2
+    fl = hipgen.flops_per_iteration(dag, 16)
+    assert fl["F_solve"] == 110 and fl["F_step"] == 64 and fl["nnz"] == 7
+    assert fl["exp_it"] == 64
+    # at most a few ops more than the reference's CSE (20 + 2 exp, 7 + 2 exp)
+    assert 22 <= fl["E_fjh"] <= 26 and fl["E_f"] == 9
+
+
+def test_parse_errors():
+    with pytest.raises(ValueError):
+        hipgen.res_jac_hes("a*x + q", ["a"], ["x"])
+    with pytest.raises(ValueError):
+        hipgen.res_jac_hes("a if x else b", ["a", "b"], ["x"])
+    with pytest.raises(ValueError):
+        hipgen.res_jac_hes("a*x", ["a", "x"], ["x"])
+
+
+def test_float_literals_are_single_precision():
+    dag = hipgen.res_jac_hes("a*exp(-x/3.5) + 0.1*a**2", ["a"], ["x"])
+    src = hipgen.emit_model(dag, "M", "float32")
+    import re
+    for lit in re.findall(r"(?<![\w.])(\d+\.\d*(?:e[-+]?\d+)?f?)", src):
+        assert lit.endswith("f"), lit
+
+
+def test_strict_math_accuracy():
+    """The reproducible transcendentals are within a few ulp of numpy's."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    hdr = os.path.join(root, "pycompute_amd", "csrc", "strict_math.h")
+    d = os.path.join(root, "oracle", "_gen")
+    os.makedirs(d, exist_ok=True)
+    cpp = os.path.join(d, "strict_probe.cpp")
+    so = os.path.join(d, "strict_probe.so")
+    with open(cpp, "w") as fh:
+        fh.write(f'#include "{hdr}"\n'
+                 'extern "C" void e32(const float* x, float* y, int n){for(int i=0;i<n;++i)y[i]=pca_exp_strict(x[i]);}\n'
+                 'extern "C" void l32(const float* x, float* y, int n){for(int i=0;i<n;++i)y[i]=pca_log_strict(x[i]);}\n'
+                 'extern "C" void e64(const double* x, double* y, int n){for(int i=0;i<n;++i)y[i]=pca_exp_strict(x[i]);}\n'
+                 'extern "C" void l64(const double* x, double* y, int n){for(int i=0;i<n;++i)y[i]=pca_log_strict(x[i]);}\n')
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", so, cpp],
+                   check=True)
+    L = ctypes.CDLL(so)
+    rng = np.random.default_rng(0)
+    for fn, dt, lo, hi, ref in (("e32", np.float32, -87, 88, np.exp), ("e64", np.float64, -700, 700, np.exp),
+                                ("l32", np.float32, -30, 30, None), ("l64", np.float64, -300, 300, None)):
+        if ref is None:
+            x = np.exp(rng.uniform(lo, hi, 20000)).astype(dt)
+            ref = np.log
+        else:
+            x = rng.uniform(lo, hi, 20000).astype(dt)
+        y = np.zeros_like(x)
+        getattr(L, fn)(x.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p), x.size)
+        exact = ref(x.astype(np.float64) if dt == np.float32 else x.astype(np.longdouble))
+        ulp = np.abs(y - exact.astype(dt)) / np.spacing(np.abs(exact.astype(dt)))
+        assert np.max(ulp) <= 4, (fn, np.max(ulp))
+
+
+@pytest.mark.parametrize("name,M,dtype,strict", [
+    ("ivim", 16, "float32", False), ("ivim", 10, "float64", False), ("t2", 32, "float32", False),
+    ("se", 16, "float32", True), ("ivim", 16, "float32", True)])
+def test_hiprtc_compiles_for_gfx950(name, M, dtype, strict):
+    from pycompute_amd import lm
+    m = MODELS[name]
+    prog = lm.get_program(lm.LMSpec(m.expr, m.pars, m.consts, M, dtype, strict=strict))
+    code = prog.module.code
+    assert code[:4] == b"\x7fELF" and b"pca_lm_fit" in code and b"gfx950" in code

@@ -1,0 +1,134 @@
+"""Host-side logic of the drop-in API (no GPU): constructor forms, consts
+layouts, dtype handling, init matrices, synthetic workloads."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from pycompute_amd import init, lm, synth
+from pycompute_amd.lsqnonlin import Expr, F, SecondOrderLevenbergMarquardt, split_consts
+from pycompute_amd.models import IVIM, MODELS, T2
+
+
+def test_constructor_forms():
+    a = SecondOrderLevenbergMarquardt(IVIM.expr, list(IVIM.pars), list(IVIM.consts), ndata=21,
+                                      dtype=np.float32, write_to_file=False)
+    b = SecondOrderLevenbergMarquardt(Expr(IVIM.expr, list(IVIM.pars), list(IVIM.consts), None,
+                                           None), None, 21, torch.float32)
+    assert a.funcid == b.funcid == "second_order_levenberg_marquardt_4_1_21_f_f7def86f0a03"
+    assert a.nparam == 4 and a.nhes == 10 and a.nconst == 1 and a.ndata == 21
+    assert a.code == b.code
+    c = SecondOrderLevenbergMarquardt(model_expr=Expr(T2.expr, T2.pars, T2.consts), f_gradf=None,
+                                      ndata=32, dtype="float64")
+    assert c.dtype == torch.float64 and c.funcid.endswith("_d_" + c.funcid.split("_")[-1])
+
+
+def test_constructor_errors():
+    with pytest.raises(RuntimeError):
+        SecondOrderLevenbergMarquardt(None, None, 10, torch.float32)
+    with pytest.raises(RuntimeError):
+        SecondOrderLevenbergMarquardt(IVIM.expr, list(IVIM.pars), list(IVIM.consts), 10,
+                                      torch.float16)
+    with pytest.raises(TypeError):
+        SecondOrderLevenbergMarquardt(IVIM.expr, list(IVIM.pars), list(IVIM.consts), 10,
+                                      ndata=10)
+
+
+def test_f_constructor_forms():
+    a = F(IVIM.expr, list(IVIM.pars), list(IVIM.consts), 21, torch.float32)
+    b = F(Expr(IVIM.expr, IVIM.pars, IVIM.consts), 21, torch.float32)
+    assert a.funcid == b.funcid == "f_4_1_21_f_f7def86f0a03"
+
+
+def test_setup_rejects_host_tensors():
+    s = SecondOrderLevenbergMarquardt(IVIM.expr, list(IVIM.pars), list(IVIM.consts), 10)
+    z = torch.zeros((4, 8))
+    with pytest.raises(ValueError):
+        s.setup(z, torch.zeros(10), torch.zeros((10, 8)), z, z)
+
+
+def test_split_consts_layouts():
+    M, N = 5, 7
+    b = torch.arange(M, dtype=torch.float32)
+    uc, pv = split_consts(b, 1, M, N)
+    assert pv is None and uc == b.tolist()
+    uc, pv = split_consts(b.reshape(1, M, 1).expand(1, M, N), 1, M, N)
+    assert pv is None and uc == b.tolist()
+    tiled = b.reshape(M, 1).repeat(1, N)
+    uc, pv = split_consts(tiled, 1, M, N)            # replicated columns detected
+    assert pv is None and uc == b.tolist()
+    uc, pv = split_consts(tiled, 1, M, N, uniform=False)
+    assert uc is None and pv is tiled
+    varying = tiled + torch.arange(N)[None, :]
+    uc, pv = split_consts(varying, 1, M, N)
+    assert uc is None and pv is varying
+    with pytest.raises(ValueError):
+        split_consts(torch.zeros(M + 1), 1, M, N)
+    uc, pv = split_consts(torch.zeros((4, M)), 4, M, N)  # (nconst, M)
+    assert pv is None and len(uc) == 4 * M
+
+
+def test_dtype_names():
+    assert lm.dtype_name(torch.float32) == "float32"
+    assert lm.dtype_name(np.float64) == "float64"
+    assert lm.dtype_name(np.dtype("float32")) == "float32"
+    assert lm.dtype_name("float") == "float32"
+    with pytest.raises(RuntimeError):
+        lm.dtype_name(torch.int32)
+
+
+def test_ivim_split():
+    lo, hi = init.ivim_split(np.arange(21) * 50.0)     # 21-point protocol: literal index split
+    assert lo.tolist() == list(range(10)) and hi.tolist() == list(range(15, 21))
+    lo, hi = init.ivim_split(synth.B_C2)
+    assert synth.B_C2[lo].max() <= 140 and synth.B_C2[hi].min() >= 500
+    lo, hi = init.ivim_split(synth.B_C1)
+    assert lo.tolist() == [0, 1, 2, 3, 4] and hi.tolist() == [7, 8, 9]
+
+
+@pytest.mark.parametrize("name", ["ivim", "t2", "se", "dki"])
+def test_init_matrix_is_least_squares(name):
+    """P log y reproduces the least-squares fits of the default initial guess."""
+    m = MODELS[name]
+    w = {"ivim": lambda: synth.ivim_c1(64, 1), "t2": lambda: synth.t2(64, 3),
+         "se": lambda: synth.se(64, 4), "dki": lambda: synth.dki(64, 6)}[name]()
+    P = init.init_matrix(m, w["consts"])
+    ly = np.log(w["data"].astype(np.float64))
+    c = P @ ly
+    if name == "ivim":
+        lo, hi = init.ivim_split(w["consts"][0])
+        b = w["consts"][0]
+        np.testing.assert_allclose(c[0:2], oracle.bpolyfit(-b[lo], ly[lo], 1), rtol=1e-9,
+                                   atol=1e-12)
+        np.testing.assert_allclose(c[2:4], oracle.bpolyfit(-b[hi], ly[hi], 1), rtol=1e-9,
+                                   atol=1e-12)
+    elif name in ("t2", "se"):
+        np.testing.assert_allclose(c, oracle.bpolyfit(-w["consts"][0], ly, 1), rtol=1e-9,
+                                   atol=1e-12)
+    else:
+        from pycompute_amd.models import dki_design
+        A = dki_design(w["consts"])
+        np.testing.assert_allclose(c, np.linalg.lstsq(A, ly, rcond=None)[0], rtol=1e-6,
+                                   atol=1e-9)
+
+
+def test_dki_noise_free_log_linear_init_is_exact():
+    from pycompute_amd.models import dki_design
+    w = synth.dki(32, seed=6)
+    A = dki_design(w["consts"])
+    y = np.exp(A @ w["truth"])
+    P = init.init_matrix(MODELS["dki"], w["consts"])
+    np.testing.assert_allclose(P @ np.log(y), w["truth"], rtol=1e-6, atol=1e-12)
+
+
+def test_synthetic_workloads_are_seeded():
+    a, b = synth.ivim_c2(1000, seed=2), synth.ivim_c2(1000, seed=2)
+    np.testing.assert_array_equal(a["data"], b["data"])
+    assert a["data"].shape == (16, 1000) and a["data"].dtype == np.float32
+    t = a["truth"]
+    assert t[0].min() >= 0.8 and t[0].max() <= 1.2 and t[1].min() >= 0.05 and t[1].max() <= 0.4
+    assert t[2].min() >= 5e-3 and t[3].max() <= 3e-3
+    d = synth.dki(100, seed=6)
+    assert d["data"].shape == (61, 100) and d["consts"].shape == (4, 61)
+    g = d["consts"][1:, 1:]
+    np.testing.assert_allclose(np.linalg.norm(g, axis=0), 1.0, rtol=1e-12)

@@ -1,0 +1,258 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+Each check is chosen so a plausible mistake in the restatement fails it:
+  * the reference's own known-answer solve (mtests/test_mats.m:49-56);
+  * numpy.linalg.solve on well-conditioned SPD systems, where GMW81 adds no
+    perturbation (a wrong pivot, scale or substitution index breaks it);
+  * GMW81 invariants on indefinite matrices: L D L^T = B + E with E diagonal
+    and >= 0, D > 0, |l_ij| sqrt(d_j) <= beta (a dropped term breaks these);
+  * the pivot the reference actually computes (permute.py:50-56 never updates
+    its running maximum): the rotation [n-1, 0, ..., n-2];
+  * closed-form 1x1 solves, model derivatives against sympy, LM invariants,
+    a noise-free fit started at the truth, FP32 vs FP64 agreement.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+# mtests/test_mats.m:49-56 (SPD A, b); the expected solution is A \ b.
+KAT_A = np.array([[1.28651749e+02, -3.17334712e-01, -2.32065395e-01, -3.49286795e-02],
+                  [-3.17334712e-01, 4.08983994e+00, 8.86404276e-01, -6.62028670e-01],
+                  [-2.32065395e-01, 8.86404276e-01, 4.78870010e+00, 1.85799852e-01],
+                  [-3.49286795e-02, -6.62028670e-01, 1.85799852e-01, 1.95811105e+00]])
+KAT_B = np.array([3.1105268, -0.01635835, -0.35105333, -0.33321792])
+KAT_X = np.array([0.02397962, -0.01595351, -0.06262880, -0.16919653])
+
+
+def test_kat_reference_matrix():
+    x64 = oracle.gmw81_solve(KAT_A, KAT_B, dtype=np.float64)
+    x32 = oracle.gmw81_solve(KAT_A, KAT_B, dtype=np.float32)
+    np.testing.assert_allclose(x64, KAT_X, rtol=2e-7, atol=1e-9)
+    np.testing.assert_allclose(x32, KAT_X, rtol=1e-5)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 12, 22])
+def test_spd_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    for _ in range(20):
+        Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+        ev = rng.uniform(0.5, 3.0, n)
+        d = np.exp(rng.uniform(-4, 4, n))           # badly scaled diagonal
+        A = (Q * ev) @ Q.T
+        A = A * np.sqrt(d)[:, None] * np.sqrt(d)[None, :]
+        b = rng.standard_normal(n)
+        np.testing.assert_allclose(oracle.gmw81_solve(A, b), np.linalg.solve(A, b),
+                                   rtol=1e-9, atol=1e-12)
+
+
+def _unpack_ldl(F):
+    n = F.shape[0]
+    L = np.tril(F, -1) + np.eye(n)
+    D = np.diag(np.diag(F))
+    return L, D
+
+
+def _gmw_constants(B):
+    n = B.shape[0]
+    eps = 4e-16
+    gamma = np.max(np.abs(np.diag(B)))
+    xi = np.max(np.abs(B[np.tril_indices(n, -1)])) if n > 1 else 0.0
+    if n > 1:
+        xi /= np.sqrt(n * n - 1)
+    return max(eps, gamma, xi)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 6, 10, 22])
+def test_gmw81_invariants_indefinite(n):
+    rng = np.random.default_rng(100 + n)
+    for _ in range(20):
+        S = rng.standard_normal((n, n))
+        B = (S + S.T) / 2 + np.diag(rng.uniform(-2, 2, n))
+        F = oracle.gmw81(B)
+        L, D = _unpack_ldl(F)
+        E = L @ D @ L.T - B
+        d = np.diag(D)
+        assert np.all(d > 0)
+        off = E - np.diag(np.diag(E))
+        assert np.max(np.abs(off)) < 1e-9 * max(1.0, np.max(np.abs(B)))
+        assert np.all(np.diag(E) >= -1e-9)
+        beta = np.sqrt(_gmw_constants(B))
+        assert np.max(np.abs(np.tril(L, -1)) * np.sqrt(d)[None, :]) <= beta * (1 + 1e-12)
+
+
+def test_gmw81_no_perturbation_when_pd_and_scaled():
+    """Unit-diagonal well-conditioned SPD: E = 0 and L D L^T = B exactly."""
+    rng = np.random.default_rng(5)
+    S = rng.standard_normal((6, 6)) * 0.1
+    B = np.eye(6) + (S + S.T) / 2
+    s = 1 / np.sqrt(np.diag(B))
+    B = B * s[:, None] * s[None, :]
+    L, D = _unpack_ldl(oracle.gmw81(B))
+    np.testing.assert_allclose(L @ D @ L.T, B, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 9, 22])
+def test_pivot_is_rotation(n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    P, perm = oracle.diag_pivot(A)
+    expect = np.array([n - 1] + list(range(n - 1)))
+    assert perm.tolist() == expect.tolist()
+    np.testing.assert_array_equal(P, A[np.ix_(expect, expect)])
+
+
+def test_nan_diagonal_gives_all_nan_step():
+    """With any NaN diagonal the reference's solution is NaN in every entry
+    (then scrubbed to a zero step), whatever the pivot; the GPU kernel relies
+    on this to apply the rotation at compile time."""
+    rng = np.random.default_rng(9)
+    for n in (2, 3, 4, 7):
+        for k in range(n):
+            S = rng.standard_normal((n, n))
+            A = S @ S.T + np.eye(n)
+            A[k, k] = np.nan
+            x = oracle.gmw81_solve(A, rng.standard_normal(n))
+            assert np.all(np.isnan(x)), (n, k, x)
+
+
+def test_one_by_one_closed_form():
+    for a, b in ((4.0, 2.0), (-4.0, 2.0), (1e-6, 3.0), (2.5, -1.0)):
+        x = oracle.gmw81_solve(np.array([[a]]), np.array([b]))
+        # scaled matrix is sign(a); GMW81 replaces it by |.| = 1
+        assert x[0] == pytest.approx(b / abs(a), rel=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# model closed forms against sympy
+# ---------------------------------------------------------------------------
+
+def _sympy_model(name):
+    import sympy as sp
+    from pycompute_amd.models import MODELS
+    m = MODELS[name]
+    syms = {s: sp.Symbol(s) for s in m.pars + m.consts}
+    expr = sp.sympify(m.expr, locals=syms)
+    return m, syms, expr
+
+
+@pytest.mark.parametrize("name", ["ivim", "t2", "se", "dki"])
+def test_oracle_model_derivatives_match_sympy(name):
+    import sympy as sp
+    from pycompute_amd import synth
+    m, syms, expr = _sympy_model(name)
+    w = {"ivim": lambda: synth.ivim_c1(16, seed=1), "t2": lambda: synth.t2(16, seed=3),
+         "se": lambda: synth.se(16, seed=4), "dki": lambda: synth.dki(4, seed=6)}[name]()
+    M = w["data"].shape[0]
+    mo, Jo, d2o = oracle.model_eval(name, w["consts"], w["truth"], M, dtype=np.float64)
+    P = [syms[p] for p in m.pars]
+    C = [syms[c] for c in m.consts]
+    f = sp.lambdify(P + C, expr, "numpy")
+    grads = [sp.diff(expr, p) for p in P]
+    hess = [sp.diff(grads[j], P[k]) for j in range(len(P)) for k in range(j + 1)]
+    fJ = [sp.lambdify(P + C, e, "numpy") for e in grads]
+    fH = [sp.lambdify(P + C, e, "numpy") for e in hess]
+    N = w["truth"].shape[1]
+    for i in range(M):
+        args = [w["truth"][j] for j in range(len(P))] + [np.full(N, w["consts"][k][i])
+                                                         for k in range(len(C))]
+        ref_m = np.broadcast_to(f(*args), (N,))
+        np.testing.assert_allclose(mo[i], ref_m, rtol=1e-12)
+        scale = np.max(np.abs(ref_m))
+        for j in range(len(P)):
+            np.testing.assert_allclose(Jo[j, i], np.broadcast_to(fJ[j](*args), (N,)),
+                                       rtol=1e-9, atol=1e-12 * scale)
+        for l in range(len(hess)):
+            np.testing.assert_allclose(d2o[l, i], np.broadcast_to(fH[l](*args), (N,)),
+                                       rtol=1e-8, atol=1e-12 * scale)
+
+
+# ---------------------------------------------------------------------------
+# LM iteration invariants
+# ---------------------------------------------------------------------------
+
+def _bounds(model, N):
+    from pycompute_amd.models import MODELS
+    m = MODELS[model]
+    return (np.repeat(np.array(m.lb)[:, None], N, 1), np.repeat(np.array(m.ub)[:, None], N, 1))
+
+
+def test_noise_free_start_at_truth_converges_in_one_iteration():
+    from pycompute_amd import synth
+    w = synth.ivim_c1(256, seed=3)
+    y = synth.ivim_signal(w["truth"], w["consts"][0])
+    lb, ub = _bounds("ivim", 256)
+    r = oracle.lm_run("ivim", w["consts"], y, lb, ub, w["truth"], 10, dtype=np.float64)
+    assert np.all(r["step_type"] == 0)
+    assert np.all(r["n_iter"] == 1)
+    np.testing.assert_array_equal(r["params"], w["truth"])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_lm_invariants(dtype):
+    from pycompute_amd import synth
+    w = synth.ivim_c1(1024, seed=1)
+    N = 1024
+    lb, ub = _bounds("ivim", N)
+    from pycompute_amd.init import ivim_split
+    lo, hi = ivim_split(w["consts"][0])
+    p0 = oracle.ivim_default_init(w["consts"][0], w["data"], lb, ub, lo, hi)
+    iters = 60
+    r = oracle.lm_run("ivim", w["consts"], w["data"], lb, ub, p0, iters, dtype=dtype, trace=True)
+    st = r["trace_st"]
+    assert set(np.unique(st).tolist()) <= {0, 1, 2, 4, 9, 10, 12}
+    lam = r["trace_lam"].astype(np.float64)
+    prev = np.vstack([np.full((1, N), 5.0), lam[:-1]])
+    normal = prev > 1e-30          # lambda may decay into subnormals, where x0.2 is inexact
+    ratio = lam[normal] / prev[normal]
+    allowed = np.array([1.0, 0.2, 2.0, 4.0, 0.4])
+    assert np.all(np.min(np.abs(ratio[..., None] / allowed - 1.0), axis=-1) < 1e-5)
+    # after a rejected step (4, 12) the parameters are unchanged, so the next
+    # iteration's SSR is bit-identical; after an accepted step it may rise,
+    # because the trial point is evaluated unclamped and then clamped
+    # (lsqnonlin.py:902-922)
+    f = r["trace_f"]
+    rejected = (st[:-1] == 4) | (st[:-1] == 12)
+    assert rejected.any()
+    np.testing.assert_array_equal(f[1:][rejected], f[:-1][rejected])
+    # bounds hold at the end, first_f is the SSR of the initial guess
+    assert np.all(r["params"] >= lb.astype(dtype)) and np.all(r["params"] <= ub.astype(dtype))
+    np.testing.assert_allclose(r["first_f"], oracle.ssr("ivim", w["consts"], w["data"],
+                                                        p0, dtype=dtype), rtol=1e-5)
+
+
+def test_resume_equals_single_run():
+    from pycompute_amd import synth
+    w = synth.t2(512, seed=3)
+    lb, ub = _bounds("t2", 512)
+    p0 = np.clip(w["truth"] * 1.2, lb, ub)
+    a = oracle.lm_run("t2", w["consts"], w["data"], lb, ub, p0, 100)
+    b = oracle.lm_run("t2", w["consts"], w["data"], lb, ub, p0, 30)
+    c = oracle.lm_run("t2", w["consts"], w["data"], lb, ub, b["params"], 70, lam=b["lam"],
+                      step_type=b["step_type"], f=b["f"])
+    np.testing.assert_array_equal(a["params"], c["params"])
+    np.testing.assert_array_equal(a["step_type"], c["step_type"])
+    np.testing.assert_array_equal(a["n_iter"], b["n_iter"] + c["n_iter"])
+
+
+def test_float32_restatement_tracks_fp64_oracle():
+    """On a well-determined model the FP32 restatement converges to the FP64
+    optimum (error against the FP64 oracle, stated per config in DESIGN.md)."""
+    from pycompute_amd import synth
+    w = synth.t2(2048, seed=3, snr=200.0)
+    lb, ub = _bounds("t2", 2048)
+    p0 = np.clip(w["truth"] * 1.1, lb, ub)
+    r32 = oracle.lm_run("t2", w["consts"], w["data"], lb, ub, p0, 100, dtype=np.float32)
+    r64 = oracle.lm_run("t2", w["consts"], w["data"], lb, ub, p0, 100, dtype=np.float64)
+    rel = np.abs(r32["params"] - r64["params"]) / np.abs(r64["params"])
+    assert np.median(rel) < 1e-5
+    assert np.percentile(rel, 99) < 1e-3
+
+
+def test_bpolyfit_restatement_is_least_squares():
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0, 1, (9, 32))
+    c = rng.standard_normal((3, 32))
+    y = c[0] + c[1] * x + c[2] * x ** 2
+    np.testing.assert_allclose(oracle.bpolyfit(x, y, 2), c, rtol=1e-9, atol=1e-9)
