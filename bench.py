@@ -2,13 +2,18 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Workload (16 b-values linspace(0, 800) s/mm^2, 128x128x64 = 1,048,576
-voxels, float32, Rician SNR ~ U[20, 100] per voxel, seeded): one step = the
-default initial guess (segmented log-linear fits, one kernel) + state reset +
-the fused LM fit (one kernel, at most 100 iterations), inputs resident in HBM.
-Prints one JSON line with the roofline of the LM kernel (algorithmic FP32
-FLOPs of the iterations actually executed / kernel time, events on the launch
-stream) and the CPU oracle timed on a sample of the same workload.
+Workload: 16 b-values linspace(0, 800) s/mm^2, 128x128x64 = 1,048,576 voxels,
+float32, Rician SNR ~ U[20, 100] per voxel, seeded. One step = one whole fit
+of every voxel with inputs resident in HBM: state reset + the default initial
+guess (segmented log-linear fits, one kernel) + the LM fit (at most 100
+iterations, chunked with convergence compaction), replayed as one hipGraph.
+
+The JSON line carries the roofline of the LM kernel: algorithmic FP32 FLOPs of
+the iterations actually executed (formula in pycompute_amd/sym/hipgen.py,
+flops_per_iteration) divided by the LM kernel time of one fit, measured with
+HIP events on the launch stream; and the CPU oracle (the FP32 restatement of
+the reference iteration, OpenMP on the host cores) timed on a sample of the
+same workload, with a parity summary against the GPU result.
 """
 from __future__ import annotations
 
@@ -23,8 +28,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "voxel fits/sec at matched final residual: MLP-init LM vs plain LM, 1M-voxel IVIM"
-PEAK_FP32 = 157.3e12   # MI355X FP32 vector (packed) FLOP/s
-PEAK_HBM = 8.0e12
+PEAK_FP32 = 157.3e12   # MI355X FP32 vector FLOP/s (packed)
 
 
 def log(*a):
@@ -49,7 +53,13 @@ def cpu_baseline(w, p0, lb, ub, gpu_params, gpu_ssr, sample, min_seconds=10.0):
         if el >= min_seconds or runs >= 50:
             break
     ssr = oracle.ssr("ivim", w["consts"], y, ref["params"], nthreads=threads)
+    ref64 = oracle.lm_run("ivim", w["consts"], y, lbs, ubs, p0s.astype(np.float64), 100,
+                          dtype=np.float64, nthreads=threads)
+    ssr64 = oracle.ssr("ivim", w["consts"], y.astype(np.float64), ref64["params"],
+                       dtype=np.float64, nthreads=threads)
     rel_f = np.abs(gpu_ssr[:sample] - ssr) / np.abs(ssr)
+    rel_f64 = np.abs(gpu_ssr[:sample] - ssr64) / np.abs(ssr64)
+    rel_o = np.abs(ssr - ssr64) / np.abs(ssr64)
     conv = (ref["step_type"] == 0)
     rel_p = (np.abs(gpu_params[:, :sample] - ref["params"]) /
              np.maximum(np.abs(ref["params"]), 1e-7)).max(axis=0)
@@ -58,9 +68,11 @@ def cpu_baseline(w, p0, lb, ub, gpu_params, gpu_ssr, sample, min_seconds=10.0):
              "sample": f"first {sample} voxels of the same workload, {runs} full fits "
                        f"(FP32 restatement of the reference iteration, OpenMP)"},
             {"sample": sample,
-             "final_ssr_rel_le_1e-4": float(np.mean(rel_f <= 1e-4)),
-             "final_ssr_rel_p99": float(np.percentile(rel_f, 99)),
-             "params_rel_le_1e-4_converged": float(np.mean(rel_p[conv] <= 1e-4)),
+             "final_ssr_rel_le_1e-4_vs_fp32_oracle": float(np.mean(rel_f <= 1e-4)),
+             "final_ssr_rel_p99_vs_fp32_oracle": float(np.percentile(rel_f, 99)),
+             "final_ssr_rel_le_1e-4_vs_fp64_oracle": float(np.mean(rel_f64 <= 1e-4)),
+             "fp32_oracle_vs_fp64_oracle_le_1e-4": float(np.mean(rel_o <= 1e-4)),
+             "params_rel_le_1e-4_oracle_converged": float(np.mean(rel_p[conv] <= 1e-4)),
              "oracle_converged": float(conv.mean())})
 
 
@@ -72,6 +84,8 @@ def main():
     ap.add_argument("--voxels", type=int, default=1 << 20)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--reference-rounding", action="store_true",
+                    help="accurate expf and correctly rounded divide (no fast math)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): voxel fits need no "
@@ -79,55 +93,84 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
-    from pycompute_amd import fit, synth
+    from pycompute_amd import autotune, fit, synth
 
     dev = torch.device("cuda:0")
     w = synth.ivim_c2(args.voxels, seed=2)
     prob = fit.make_problem("ivim", w["consts"], w["data"], device=dev)
     N, M = prob.N, prob.M
-    solver = fit.make_solver(prob)
-    pars = torch.empty((4, N), dtype=torch.float32, device=dev)
-    fit.default_init(prob, out=pars)
-    solver.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
-    stream = torch.cuda.current_stream(dev)
-    flops = solver.program.flops
-
-    def step(ev=None):
-        fit.default_init(prob, out=pars)
-        solver.lam_t.fill_(5.0)
-        solver.step_type_t.fill_(1)
-        solver.n_iter_t.zero_()
-        if ev is not None:
-            ev[0].record(stream)
-        solver.run(100)
-        if ev is not None:
-            ev[1].record(stream)
-
+    p0 = fit.default_init(prob)
     log(f"workload: IVIM, N={N}, M={M}, warmup {args.warmup}, steps {args.steps}")
+
+    def setup(s):
+        pars = p0.clone()
+        s.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+
+    def restore(s):
+        s.pars_t.copy_(p0)
+        s.reset_state()
+
+    def make(**kn):
+        s = fit.make_solver(prob, **kn)
+        setup(s)
+        return s
+
+    knobs = autotune.tune(make, restore, prob.model.expr, M, "float32",
+                          allow_fast=not args.reference_rounding, log=log)
+    log(f"tuned: {knobs}")
+    solver = make(compaction=True, **knobs)
+    flops = solver.program.flops
+    pars = solver.pars_t
+    stream = torch.cuda.current_stream(dev)
+
+    # one whole fit = reset + default init + chunked LM, captured once
+    solver.capture_graph(100, init=lambda st: fit._init.default_init(
+        prob.model, prob.consts, prob.data_t, prob.lb_t, prob.ub_t, out=pars, stream=st))
     for _ in range(args.warmup):
-        step()
+        solver.replay_graph()
     torch.cuda.synchronize()
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(events[k])
+    for _ in range(args.steps):
+        solver.replay_graph()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
-    k_ms = [a.elapsed_time(b) for a, b in events]
-    k_avg = sum(k_ms) / len(k_ms)
 
     n_iter = solver.n_iter_t[0].to(torch.int64)
     conv = (solver.step_type_t[0] == 0).to(torch.int64)
     it_sum = int(n_iter.sum())
     conv_sum = int(conv.sum())
+    gpu_params = solver.pars_t.cpu().numpy()
+    gpu_ssr = solver.final_f().cpu().numpy()[0]
+
+    # LM kernel time of one fit (the chunk launches, events on the launch stream)
+    k_ms = []
+    for _ in range(max(3, args.steps // 2)):
+        restore(solver)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        solver.run(100)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        k_ms.append(e0.elapsed_time(e1))
+    k_avg = sum(k_ms) / len(k_ms)
     alg_flops = (it_sum - conv_sum) * flops["F_it"] + conv_sum * flops["F_conv"]
     achieved = alg_flops / (k_avg * 1e-3)
-    log(f"LM kernel {k_avg:.3f} ms avg, {it_sum / N:.2f} iterations/voxel, converged "
-        f"{conv_sum / N:.4f}, {achieved / 1e12:.2f} TFLOP/s")
+    log(f"fit {ms_step:.3f} ms/step; LM kernels {k_avg:.3f} ms per fit, {it_sum / N:.2f} "
+        f"iterations/voxel, converged {conv_sum / N:.4f}, {achieved / 1e12:.2f} TFLOP/s")
+
+    # the single-launch kernel (no compaction) on the same fit, for reference
+    plain = make(compaction=False, **{k: v for k, v in knobs.items() if k != "chunk"})
+    pk = []
+    for _ in range(3):
+        restore(plain)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        plain.run(100)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        pk.append(e0.elapsed_time(e1))
 
     result = {
         "metric": METRIC, "value": N / (ms_step * 1e-3), "unit": "voxel-fits/s", "n_gpus": 1,
@@ -137,22 +180,20 @@ def main():
         "config": {"workload": "IVIM 1,048,576 voxels (128x128x64), 16 b-values 0-800, plain LM "
                                "(reference default init), max 100 iterations",
                    "voxels": N, "measurements": M, "max_iters": 100, "method": "plain LM",
-                   "init": "default segmented log-linear"},
+                   "init": "default segmented log-linear", "tuning": knobs,
+                   "execution": "hipGraph: reset + init + chunked LM with compaction"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "pca_lm_fit", "kernel_ms": k_avg,
-                     "flops_per_iteration": flops["F_it"], "exp_per_iteration": flops["exp_it"],
-                     "iterations_per_voxel": it_sum / N, "converged_frac": conv_sum / N},
+                     "kernel": "pca_lm_chunk (all chunk launches of one fit)",
+                     "kernel_ms": k_avg, "flops_per_iteration": flops["F_it"],
+                     "exp_per_iteration": flops["exp_it"],
+                     "iterations_per_voxel": it_sum / N, "converged_frac": conv_sum / N,
+                     "single_launch_kernel_ms": min(pk)},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        fit.default_init(prob, out=pars)
-        p0 = pars.cpu().numpy()
-        step()
-        torch.cuda.synchronize()
-        gp = solver.pars_t.cpu().numpy()
-        gf = solver.final_f().cpu().numpy()[0]
-        cb, par = cpu_baseline(w, p0, prob.lb_t.cpu().numpy(), prob.ub_t.cpu().numpy(), gp, gf,
+        cb, par = cpu_baseline(w, p0.cpu().numpy(), prob.lb_t.cpu().numpy(),
+                               prob.ub_t.cpu().numpy(), gpu_params, gpu_ssr,
                                min(args.cpu_sample, N))
         result["cpu_baseline"] = cb
         result["parity"] = par

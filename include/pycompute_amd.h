@@ -95,8 +95,34 @@ typedef struct pca_eval_args {
   int64_t N;
 } pca_eval_args;
 
+/* Packed working set of the chunked (compacting) fit: entry e of `cap`
+ * entries, arrays (rows, cap) with the module's element type. */
+typedef struct pca_ws {
+  void* y;          /* (M, cap) measurements */
+  void* cv;         /* (nconst*M, cap) per-voxel consts, or NULL */
+  void* p;          /* (n, cap) */
+  void* lo;         /* (n, cap) */
+  void* hi;         /* (n, cap) */
+  void* lam;        /* (cap) */
+  void* f;          /* (cap) */
+  void* f_first;    /* (cap) */
+  int32_t* idx;     /* (cap) original voxel index */
+  int32_t* cnt;     /* (cap) iterations executed in this run */
+  int8_t* st;       /* (cap) */
+  int64_t cap;
+} pca_ws;
+
+/* One model's part of a chunked launch: its caller arrays, its voxel-uniform
+ * consts (device pointer, NULL for per-voxel consts) and two queues. */
+typedef struct pca_model_job {
+  pca_lm_args a;
+  const void* uc;
+  pca_ws ws[2];
+} pca_model_job;
+
 typedef struct pca_module_s* pca_module;
 typedef struct pca_kernel_s* pca_kernel;
+typedef struct pca_graph_s* pca_graph;
 
 /* Message of the last failure on this thread ("" if none). */
 const char* pca_last_error(void);
@@ -132,6 +158,26 @@ int pca_module_kernel(pca_module module, const char* name, pca_kernel* out);
  * (pycompute/cuda/lsqnonlin.py:1058-1073). */
 int pca_lm_launch(pca_kernel kernel, const pca_lm_args* args, const void* uconsts,
                   size_t uconsts_bytes, uint32_t block, void* stream);
+
+/* Chunked fit with convergence compaction over `nmodels` models (kernels
+ * "pca_lm_chunk" and "pca_reset_counts" of one generated module): resets the
+ * queue counters (device array of (n_chunks+1)*nmodels int32) with a kernel,
+ * then launches one chunk per entry of chunk_iters (host array; their sum is
+ * the iteration count) on a grid of `grid` blocks. Per-voxel results equal
+ * those of pca_lm_launch with the same iteration count. Replaces the same
+ * per-iteration launch loop (pycompute/cuda/lsqnonlin.py:1058-1073) and, for
+ * several models, that many separate fits. Capturable in a graph. */
+int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
+                          const pca_model_job* jobs, int32_t nmodels, int32_t* counts,
+                          const int32_t* chunk_iters, int32_t n_chunks, uint32_t grid,
+                          uint32_t block, void* stream);
+
+/* hipGraph capture of everything launched on `stream` between begin and
+ * end (no counterpart in the reference, which launches from a Python loop). */
+int pca_graph_begin(void* stream);
+int pca_graph_end(void* stream, pca_graph* out);
+int pca_graph_launch(pca_graph graph, void* stream);
+int pca_graph_destroy(pca_graph graph);
 
 /* Launch the standalone SSR kernel ("pca_ssr"). Replaces F.run
  * (pycompute/cuda/lsqnonlin.py:137-145). */

@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must be loaded before the HIP library, see above)
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libpycompute_amd.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 PCA_UC_MAX = 256
 
@@ -40,6 +40,16 @@ class SSRArgs(ctypes.Structure):
 class EvalArgs(ctypes.Structure):
     _fields_ = [("params", _P), ("consts", _P), ("m", _P), ("J", _P), ("d2", _P),
                 ("N", ctypes.c_int64)]
+
+
+class WS(ctypes.Structure):
+    _fields_ = [("y", _P), ("cv", _P), ("p", _P), ("lo", _P), ("hi", _P), ("lam", _P),
+                ("f", _P), ("f_first", _P), ("idx", _P), ("cnt", _P), ("st", _P),
+                ("cap", ctypes.c_int64)]
+
+
+class ModelJob(ctypes.Structure):
+    _fields_ = [("a", LMArgs), ("uc", _P), ("ws", WS * 2)]
 
 
 class PcaError(RuntimeError):
@@ -73,9 +83,17 @@ def lib():
         fn.argtypes = [_P, ctypes.POINTER(st), _P, ctypes.c_size_t, ctypes.c_uint32, _P]
     L.pca_launch.argtypes = [_P] + [ctypes.c_uint32] * 7 + [_P, ctypes.POINTER(_P)]
     L.pca_device_arch.argtypes = [ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32]
+    L.pca_lm_chunked_launch.argtypes = [_P, _P, ctypes.POINTER(ModelJob), ctypes.c_int32, _P,
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                                        ctypes.c_uint32, ctypes.c_uint32, _P]
+    L.pca_graph_begin.argtypes = [_P]
+    L.pca_graph_end.argtypes = [_P, ctypes.POINTER(_P)]
+    L.pca_graph_launch.argtypes = [_P, _P]
+    L.pca_graph_destroy.argtypes = [_P]
     for name in ("pca_compile", "pca_module_load", "pca_module_unload", "pca_module_kernel",
                  "pca_lm_launch", "pca_ssr_launch", "pca_eval_launch", "pca_launch",
-                 "pca_device_arch"):
+                 "pca_device_arch", "pca_lm_chunked_launch", "pca_graph_begin",
+                 "pca_graph_end", "pca_graph_launch", "pca_graph_destroy"):
         getattr(L, name).restype = ctypes.c_int
     if L.pca_abi_version() != ABI_VERSION:
         raise PcaError(f"{LIB_PATH} has ABI {L.pca_abi_version()}, expected {ABI_VERSION}; rebuild")

@@ -1,0 +1,146 @@
+"""Chunked LM execution with convergence compaction (kernel pca_lm_chunk).
+
+The iterations are split into chunks; after each chunk every wavefront moves
+its still-active voxels into a packed queue with one atomic, so the next chunk
+runs full wavefronts of unconverged voxels only. Per-voxel results are
+identical to the single-launch kernel (tested on the GPU). The sequence of
+chunk launches has a fixed shape, so it can be captured once in a hipGraph and
+replayed.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .lm import ptr
+
+
+def chunk_schedule(iters: int, k: int) -> list:
+    """Chunks of k iterations covering `iters` (the last one shorter)."""
+    if iters <= 0:
+        return []
+    k = max(1, int(k))
+    out = [k] * (iters // k)
+    if iters % k:
+        out.append(iters % k)
+    return out
+
+
+class Workset:
+    """Two packed queues (ping-pong) for one model's voxels."""
+
+    def __init__(self, N, M, n, ncm_per_voxel, dtype, device):
+        self.cap = max(int(N), 1)
+        kw = dict(dtype=dtype, device=device)
+        self.tensors = []
+        self.structs = []
+        for _ in range(2):
+            t = {"y": torch.empty((M, self.cap), **kw),
+                 "cv": torch.empty((ncm_per_voxel, self.cap), **kw) if ncm_per_voxel else None,
+                 "p": torch.empty((n, self.cap), **kw), "lo": torch.empty((n, self.cap), **kw),
+                 "hi": torch.empty((n, self.cap), **kw), "lam": torch.empty(self.cap, **kw),
+                 "f": torch.empty(self.cap, **kw), "f_first": torch.empty(self.cap, **kw),
+                 "idx": torch.empty(self.cap, dtype=torch.int32, device=device),
+                 "cnt": torch.empty(self.cap, dtype=torch.int32, device=device),
+                 "st": torch.empty(self.cap, dtype=torch.int8, device=device)}
+            self.tensors.append(t)
+            self.structs.append(_lib.WS(**{k: ptr(v) for k, v in t.items()}, cap=self.cap))
+
+
+class ChunkedRun:
+    """One or several models' fits as one chunked launch sequence.
+
+    jobs: list of (program, LMArgs, uniform-consts tensor or None, Workset).
+    All programs must be the same module (one pca_lm_chunk kernel).
+    """
+
+    def __init__(self, program, jobs, device, block: int, grid: int | None = None):
+        self.program = program
+        self.jobs = jobs
+        self.block = block
+        n_total = sum(int(a.N) for _, a, _, _ in jobs)
+        waves = (n_total + 63) // 64
+        self.grid = grid or max(1, min((waves * 64 + block - 1) // block, 8192))
+        arr = (_lib.ModelJob * len(jobs))()
+        for i, (_, a, uc, ws) in enumerate(jobs):
+            arr[i].a = a
+            arr[i].uc = ptr(uc)
+            arr[i].ws[0] = ws.structs[0]
+            arr[i].ws[1] = ws.structs[1]
+        self.job_array = arr
+        self.device = device
+        self._counts = None
+        self._graph = None
+        self._graph_key = None
+
+    def _counts_for(self, n_chunks):
+        need = (n_chunks + 1) * len(self.jobs)
+        if self._counts is None or self._counts.numel() < need:
+            self._counts = torch.zeros(need, dtype=torch.int32, device=self.device)
+        return self._counts
+
+    def set_iters(self, iters: int):
+        for i in range(len(self.jobs)):
+            self.job_array[i].a.iters = int(iters)
+
+    def launch(self, schedule, stream=None):
+        if not schedule:
+            return
+        counts = self._counts_for(len(schedule))
+        s = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
+        k = (ctypes.c_int32 * len(schedule))(*schedule)
+        L = _lib.lib()
+        _lib.check(L.pca_lm_chunked_launch(self.program.kernel("pca_lm_chunk"),
+                                           self.program.kernel("pca_reset_counts"),
+                                           self.job_array, len(self.jobs), counts.data_ptr(), k,
+                                           len(schedule), self.grid, self.block, s),
+                   "pca_lm_chunked_launch")
+
+    # -- hipGraph -------------------------------------------------------------
+    def capture(self, schedule, prologue=None):
+        """Capture prologue() (e.g. the init kernel) and the chunk sequence into
+        a graph on a side stream; returns the graph handle."""
+        self.destroy_graph()
+        self._counts_for(len(schedule))
+        self.program.kernel("pca_lm_chunk")
+        self.program.kernel("pca_reset_counts")
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        L = _lib.lib()
+        with torch.cuda.stream(side):
+            h = side.cuda_stream
+            _lib.check(L.pca_graph_begin(h), "pca_graph_begin")
+            ok = False
+            try:
+                if prologue is not None:
+                    prologue(h)
+                self.launch(schedule, stream=h)
+                ok = True
+            finally:
+                g = ctypes.c_void_p()
+                rc = L.pca_graph_end(h, ctypes.byref(g))
+                if ok:
+                    _lib.check(rc, "pca_graph_end")
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        self._graph = g
+        self._graph_key = tuple(schedule)
+        return g
+
+    def replay(self, stream=None):
+        if self._graph is None:
+            raise RuntimeError("no graph captured")
+        s = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
+        _lib.check(_lib.lib().pca_graph_launch(self._graph, s), "pca_graph_launch")
+
+    def destroy_graph(self):
+        if self._graph is not None:
+            _lib.lib().pca_graph_destroy(self._graph)
+            self._graph = None
+
+    def __del__(self):
+        try:
+            self.destroy_graph()
+        except Exception:
+            pass

@@ -1,12 +1,13 @@
 // Fused batched Levenberg-Marquardt fit: every LM iteration of a voxel runs in
-// one lane, in registers, within a single launch.
+// one lane, in registers.
 //
-// Template: the code generator (pycompute_amd/sym/hipgen.py) concatenates
-// prelude.hip, a model struct typedef'd to `Model` (NP, NC, NH, resid, accum,
-// fjh) and this file, with PCA_M (measurements) and PCA_BLOCK (threads per
-// block) defined; hiprtc compiles it for gfx950.
+// Template: the code generator (pycompute_amd/lm.py) concatenates
+// strict_math.h, prelude.hip, the generated model structs (Model0 ..
+// Model{PCA_NMODELS-1}, with Model = Model0) and this file, with PCA_M
+// (measurements of Model0), PCA_M0.. and PCA_BLOCK defined; hiprtc compiles
+// the result for gfx950.
 //
-// One launch of pca_lm_fit performs `iters` launches of the reference kernel
+// pca_lm_fit: one launch performs `iters` launches of the reference kernel
 // k_second_order_levenberg_marquardt (pycompute/cuda/lsqnonlin.py:949-1073,
 // emitted test/ivim/second_order_levenberg_marquardt_4_1_21_f_f7def86f0a03.cu:416-535).
 // Voxel v iterates while step_type != 0, in the reference's order:
@@ -15,10 +16,17 @@
 // State (params, lambda, step_type, f) stays in VGPRs between iterations and
 // is read and written once per launch, so calling run() again resumes.
 //
+// pca_lm_chunk: the same iteration, run as a sequence of chunks of k
+// iterations over compacted work queues (one per model). After each chunk a
+// wavefront appends its still-active voxels to the next queue with one atomic
+// per wave; converged voxels write their results and leave. Waves therefore
+// stay full of active voxels instead of idling behind the slowest lane, and
+// several models (different n, M) share one persistent launch. Per-voxel
+// results are identical to pca_lm_fit.
+//
 // Layout: per-voxel arrays are structure-of-arrays (rows, N), voxel index
-// fastest, so each wavefront's loads and stores are unit-stride. Voxel-uniform
-// consts arrive as a kernel argument block (scalar registers after unrolling);
-// per-voxel consts are read from (NC*M, N).
+// fastest, so wavefront loads and stores are unit-stride; the working sets of
+// the chunked path are packed the same way.
 
 struct pca_lm_args {
   const real* data;      // (M, N)
@@ -51,6 +59,18 @@ struct pca_uconsts {
 
 #ifndef PCA_UNROLL_M
 #define PCA_UNROLL_M PCA_M
+#endif
+// Occupancy hint (waves per SIMD) chosen by the autotuner; caps VGPRs per lane.
+#ifdef PCA_WAVES_PER_EU
+#define PCA_KATTR __attribute__((amdgpu_waves_per_eu(PCA_WAVES_PER_EU, PCA_WAVES_PER_EU)))
+#else
+#define PCA_KATTR
+#endif
+#ifndef PCA_NMODELS
+#define PCA_NMODELS 1
+typedef Model Model0;
+#define PCA_M0 PCA_M
+#define PCA_UNROLL_M0 PCA_UNROLL_M
 #endif
 
 namespace pca {
@@ -163,178 +183,449 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
   for (int i = 0; i < n; ++i) s[perm(n, i)] = z[i] / sc[i];
 }
 
-}  // namespace pca
+struct Settings {
+  real tol, mu, eta, acc, dec;
+};
 
-static_assert(Model::NC * PCA_M <= PCA_UC_MAX, "consts exceed the kernel-argument block");
+__device__ __forceinline__ Settings settings(const pca_lm_args& a) {
+  return Settings{(real)a.tol, (real)a.mu, (real)a.eta, (real)a.acc, (real)a.dec};
+}
 
-extern "C" __global__ void __launch_bounds__(PCA_BLOCK)
-pca_lm_fit(pca_lm_args a, pca_uconsts uc) {
-  constexpr int NP = Model::NP, NC = Model::NC, NH = Model::NH, M = PCA_M;
-  const long long N = a.N;
-  const long long v = (long long)blockIdx.x * PCA_BLOCK + threadIdx.x;
-  if (v >= N) return;
-
-  const real tol = (real)a.tol, mu = (real)a.mu, eta = (real)a.eta;
-  const real acc = (real)a.acc, dec = (real)a.dec;
-
+// One voxel's state in registers.
+template <class Mdl, int M>
+struct Voxel {
+  static constexpr int NP = Mdl::NP, NH = Mdl::NH;
   real y[M];
-#pragma unroll
-  for (int i = 0; i < M; ++i) y[i] = a.data[(long long)i * N + v];
   real p[NP], lo[NP], hi[NP];
+  real lam, f, f_first;
+  int st;
+  real g[NP], h[NH], hld[NP], s[NP];
+};
+
+// One reference launch of the LM iteration for a voxel with st != 0
+// (lsqnonlin.py:820-947). cf(k, i) returns const k of measurement i.
+// Returns after the convergence test when the voxel converges (st = 0).
+template <class Mdl, int M, int UNROLL, class CF>
+__device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, const CF& cf) {
+  constexpr int NP = Mdl::NP, NC = Mdl::NC, NH = Mdl::NH;
+  // fghhl (lsqnonlin.py:206-346)
+  V.f = 0;
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
-    p[j] = a.params[(long long)j * N + v];
-    lo[j] = a.lb[(long long)j * N + v];
-    hi[j] = a.ub[(long long)j * N + v];
+    V.g[j] = 0;
+    V.hld[j] = 0;
   }
-#ifdef PCA_CONSTS_PER_VOXEL
-  real cv[NC * M];
 #pragma unroll
-  for (int q = 0; q < NC * M; ++q) cv[q] = a.consts[(long long)q * N + v];
-#define PCA_CONST(k, i) cv[(k) * M + (i)]
-#else
-#define PCA_CONST(k, i) uc.c[(k) * M + (i)]
-#endif
+  for (int l = 0; l < NH; ++l) V.h[l] = 0;
+#pragma unroll UNROLL
+  for (int i = 0; i < M; ++i) {
+    real c[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+    Mdl::accum(V.p, c, V.y[i], V.lam, V.f, V.g, V.h, V.hld);
+  }
 
-  real lam = a.lam[v];
-  int st = a.step_type[v];
-  real f = a.f[v];
-  real f_first = f;
-  real g[NP], h[NH], hld[NP], s[NP];
-  int cnt = 0;
+  // step: hl s = g, then scrub NaN/Inf (lsqnonlin.py:843-891)
+  gmw81_solve<NP>(V.h, V.hld, V.g, V.s);
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+    if (pca_nonfinite(V.s[j])) V.s[j] = 0;
 
-  for (int it = 0; it < a.iters && st != 0; ++it) {
-    ++cnt;
-    // fghhl (lsqnonlin.py:206-346)
-    f = 0;
-#pragma unroll
-    for (int j = 0; j < NP; ++j) { g[j] = 0; hld[j] = 0; }
-#pragma unroll
-    for (int l = 0; l < NH; ++l) h[l] = 0;
-#pragma unroll PCA_UNROLL_M
-    for (int i = 0; i < M; ++i) {
-      real c[NC];
-#pragma unroll
-      for (int k = 0; k < NC; ++k) c[k] = PCA_CONST(k, i);
-      Model::accum(p, c, y[i], lam, f, g, h, hld);
-    }
-    if (it == 0) f_first = f;
-
-    // step: hl s = g, then scrub NaN/Inf (lsqnonlin.py:843-891)
-    pca::gmw81_solve<NP>(h, hld, g, s);
-#pragma unroll
-    for (int j = 0; j < NP; ++j)
-      if (pca_nonfinite(s[j])) s[j] = 0;
-
-    // projected-gradient convergence (lsqnonlin.py:691-729)
-    {
-      bool clamped = false;
-      real norm = 0;
-#pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        real t = p[j] - g[j];
-        if (t > hi[j]) {
-          clamped = true;
-          t = hi[j];
-        } else if (t < lo[j]) {
-          clamped = true;
-          t = lo[j];
-        }
-        t = p[j] - t;
-        norm += t * t;
-      }
-      if (norm < tol * ((real)1 + f) && ((st & 1) || clamped)) {
-        st = 0;
-        break;
-      }
-    }
-
-    // trial point and its SSR (unclamped, lsqnonlin.py:902-911)
-    real ptp[NP];
-#pragma unroll
-    for (int j = 0; j < NP; ++j) ptp[j] = p[j] - s[j];
-    real ftp = 0;
-#pragma unroll PCA_UNROLL_M
-    for (int i = 0; i < M; ++i) {
-      real c[NC];
-#pragma unroll
-      for (int k = 0; k < NC; ++k) c[k] = PCA_CONST(k, i);
-      const real r = Model::resid(ptp, c, y[i]);
-      ftp += r * r;
-    }
-
-    // gain ratio and step type (lsqnonlin.py:432-498)
-    const real actual = (real)0.5 * (f - ftp);
-    real pred = 0;
-#pragma unroll
-    for (int i = 0; i < NP; ++i)
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        const real e = h[pca::tri(i, j)] * s[i] * s[j];
-        if (i == j)
-          pred -= e;
-        else
-          pred -= (real)2 * e;
-      }
-    pred *= (real)0.5;
-#pragma unroll
-    for (int i = 0; i < NP; ++i) pred += s[i] * g[i];
-    const real rho = actual / pred;
-    if (rho > mu && actual > 0) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) p[j] = ptp[j];
-      if (rho > eta) {
-        lam *= acc;
-        st = 1;
-      } else {
-        st = 2;
-      }
-    } else {
-      lam *= dec;
-      st = 4;
-    }
-    if (pred < 0) {
-      lam *= dec;
-      st |= 8;
-    }
-
-    // clamp (lsqnonlin.py:584-607)
+  // projected-gradient convergence (lsqnonlin.py:691-729)
+  {
+    bool clamped = false;
+    real norm = 0;
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
-      if (p[j] > hi[j])
-        p[j] = hi[j];
-      else if (p[j] < lo[j])
-        p[j] = lo[j];
+      real t = V.p[j] - V.g[j];
+      if (t > V.hi[j]) {
+        clamped = true;
+        t = V.hi[j];
+      } else if (t < V.lo[j]) {
+        clamped = true;
+        t = V.lo[j];
+      }
+      t = V.p[j] - t;
+      norm += t * t;
+    }
+    if (norm < S.tol * ((real)1 + V.f) && ((V.st & 1) || clamped)) {
+      V.st = 0;
+      return;
     }
   }
-#undef PCA_CONST
 
+  // trial point and its SSR (unclamped, lsqnonlin.py:902-911)
+  real ptp[NP];
 #pragma unroll
-  for (int j = 0; j < NP; ++j) a.params[(long long)j * N + v] = p[j];
-  a.lam[v] = lam;
-  a.step_type[v] = (signed char)st;
-  a.f[v] = f;
-  if (a.first_f) a.first_f[v] = f_first;
-  if (a.last_f) a.last_f[v] = f;
-  if (a.n_iter) a.n_iter[v] += cnt;
-  if (cnt > 0) {
+  for (int j = 0; j < NP; ++j) ptp[j] = V.p[j] - V.s[j];
+  real ftp = 0;
+#pragma unroll UNROLL
+  for (int i = 0; i < M; ++i) {
+    real c[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+    const real r = Mdl::resid(ptp, c, V.y[i]);
+    ftp += r * r;
+  }
+
+  // gain ratio and step type (lsqnonlin.py:432-498)
+  const real actual = (real)0.5 * (V.f - ftp);
+  real pred = 0;
+#pragma unroll
+  for (int i = 0; i < NP; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      const real e = V.h[tri(i, j)] * V.s[i] * V.s[j];
+      if (i == j)
+        pred -= e;
+      else
+        pred -= (real)2 * e;
+    }
+  pred *= (real)0.5;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) pred += V.s[i] * V.g[i];
+  const real rho = actual / pred;
+  if (rho > S.mu && actual > 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) V.p[j] = ptp[j];
+    if (rho > S.eta) {
+      V.lam *= S.acc;
+      V.st = 1;
+    } else {
+      V.st = 2;
+    }
+  } else {
+    V.lam *= S.dec;
+    V.st = 4;
+  }
+  if (pred < 0) {
+    V.lam *= S.dec;
+    V.st |= 8;
+  }
+
+  // clamp (lsqnonlin.py:584-607)
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    if (V.p[j] > V.hi[j])
+      V.p[j] = V.hi[j];
+    else if (V.p[j] < V.lo[j])
+      V.p[j] = V.lo[j];
+  }
+}
+
+// Write a voxel's results to the caller's arrays (original index v).
+template <class Mdl, int M>
+__device__ __forceinline__ void store_result(const pca_lm_args& a, long long v,
+                                             const Voxel<Mdl, M>& V, int cnt_total,
+                                             bool have_workspace) {
+  constexpr int NP = Mdl::NP;
+  const long long N = a.N;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) a.params[(long long)j * N + v] = V.p[j];
+  a.lam[v] = V.lam;
+  a.step_type[v] = (signed char)V.st;
+  a.f[v] = V.f;
+  if (a.first_f) a.first_f[v] = V.f_first;
+  if (a.last_f) a.last_f[v] = V.f;
+  if (a.n_iter) a.n_iter[v] += cnt_total;
+  if (have_workspace) {
     if (a.g_out) {
 #pragma unroll
-      for (int j = 0; j < NP; ++j) a.g_out[(long long)j * N + v] = g[j];
+      for (int j = 0; j < NP; ++j) a.g_out[(long long)j * N + v] = V.g[j];
     }
     if (a.step_out) {
 #pragma unroll
-      for (int j = 0; j < NP; ++j) a.step_out[(long long)j * N + v] = s[j];
+      for (int j = 0; j < NP; ++j) a.step_out[(long long)j * N + v] = V.s[j];
     }
     if (a.h_out || a.hl_out) {
 #pragma unroll
       for (int j = 0; j < NP; ++j)
 #pragma unroll
         for (int k = 0; k <= j; ++k) {
-          const int l = pca::tri(j, k);
-          if (a.h_out) a.h_out[(long long)l * N + v] = h[l];
-          if (a.hl_out) a.hl_out[(long long)l * N + v] = (j == k) ? hld[j] : h[l];
+          const int l = tri(j, k);
+          if (a.h_out) a.h_out[(long long)l * N + v] = V.h[l];
+          if (a.hl_out) a.hl_out[(long long)l * N + v] = (j == k) ? V.hld[j] : V.h[l];
         }
     }
+  }
+}
+
+template <class Mdl, int M>
+__device__ __forceinline__ void load_user(const pca_lm_args& a, long long v, Voxel<Mdl, M>& V) {
+  const long long N = a.N;
+#pragma unroll
+  for (int i = 0; i < M; ++i) V.y[i] = a.data[(long long)i * N + v];
+#pragma unroll
+  for (int j = 0; j < Mdl::NP; ++j) {
+    V.p[j] = a.params[(long long)j * N + v];
+    V.lo[j] = a.lb[(long long)j * N + v];
+    V.hi[j] = a.ub[(long long)j * N + v];
+  }
+  V.lam = a.lam[v];
+  V.st = a.step_type[v];
+  V.f = a.f[v];
+  V.f_first = V.f;
+}
+
+}  // namespace pca
+
+#ifndef PCA_CONSTS_PER_VOXEL
+static_assert(Model::NC * PCA_M <= PCA_UC_MAX, "consts exceed the kernel-argument block");
+#endif
+
+// Per-voxel consts: kept in registers when there are at most this many.
+#define PCA_CV_REGS 64
+
+extern "C" __global__ void __launch_bounds__(PCA_BLOCK) PCA_KATTR
+pca_lm_fit(pca_lm_args a, pca_uconsts uc) {
+  constexpr int NC = Model::NC, M = PCA_M;
+  const long long N = a.N;
+  const long long v = (long long)blockIdx.x * PCA_BLOCK + threadIdx.x;
+  if (v >= N) return;
+  const pca::Settings S = pca::settings(a);
+  pca::Voxel<Model, M> V;
+  pca::load_user(a, v, V);
+#ifdef PCA_CONSTS_PER_VOXEL
+  constexpr int NCV = (NC * M <= PCA_CV_REGS) ? NC * M : 1;
+  real cv[NCV];
+  if constexpr (NC * M <= PCA_CV_REGS) {
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * N + v];
+  }
+  const real* __restrict__ cg = a.consts;
+  auto cf = [&](int k, int i) {
+    if constexpr (NC * M <= PCA_CV_REGS)
+      return cv[(k * M + i) % NCV];
+    else
+      return cg[(long long)(k * M + i) * N + v];
+  };
+#else
+  auto cf = [&](int k, int i) { return uc.c[k * M + i]; };
+#endif
+  int cnt = 0;
+  for (int it = 0; it < a.iters && V.st != 0; ++it) {
+    ++cnt;
+    pca::iterate<Model, M, PCA_UNROLL_M>(V, S, cf);
+    if (it == 0) V.f_first = V.f;
+  }
+  pca::store_result(a, v, V, cnt, cnt > 0);
+}
+
+// ---------------------------------------------------------------------------
+// chunked execution with convergence compaction
+// ---------------------------------------------------------------------------
+
+struct pca_ws {          // packed working set, entry e of capacity cap
+  real* y;               // (M, cap)
+  real* cv;              // (NC*M, cap) per-voxel consts, or null
+  real* p;               // (NP, cap)
+  real* lo;              // (NP, cap)
+  real* hi;              // (NP, cap)
+  real* lam;             // (cap)
+  real* f;               // (cap)
+  real* f_first;         // (cap)
+  int* idx;              // (cap) original voxel index
+  int* cnt;              // (cap) iterations executed in this run
+  signed char* st;       // (cap)
+  long long cap;
+};
+
+struct pca_model_job {
+  pca_lm_args a;         // the caller's arrays for this model
+  const real* uc;        // voxel-uniform consts (device), or null for per-voxel consts
+  pca_ws ws[2];          // ping-pong queues
+};
+
+struct pca_chunk_args {
+  pca_model_job job[PCA_NMODELS];
+  int* counts;           // (n_chunks + 1, NMODELS): queue length read by chunk c
+  int chunk;             // chunk c reads queue c & 1 (counts row c), appends to the other
+  int it0;               // iteration index of the chunk's first iteration
+  int k;                 // iterations in this chunk
+  int last;              // 1: no chunk follows; every voxel stores its result
+};
+
+namespace pca {
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+template <int MI, class Mdl, int M, int UNROLL>
+__device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e, bool valid) {
+  constexpr int NP = Mdl::NP, NC = Mdl::NC;
+  const pca_model_job& J = A.job[MI];
+  const pca_lm_args& a = J.a;
+  const pca_ws& in = J.ws[A.chunk & 1];
+  const pca_ws& out = J.ws[(A.chunk + 1) & 1];
+  const Settings S = settings(a);
+  const bool from_user = (A.chunk == 0);
+#ifdef PCA_CONSTS_PER_VOXEL
+  constexpr bool per_voxel = true;
+  // Per-voxel consts travel with the working set only when they fit in
+  // registers; the host sends larger per-voxel layouts to pca_lm_fit instead.
+  static_assert(NC * M <= PCA_CV_REGS, "per-voxel consts too large for the chunked kernel");
+#else
+  constexpr bool per_voxel = false;
+#endif
+
+  Voxel<Mdl, M> V;
+  long long v = 0;
+  int cnt_total = 0;
+  constexpr int NCV = (NC * M <= PCA_CV_REGS) ? NC * M : 1;
+  real cv[NCV];
+  if (valid) {
+    if (from_user) {
+      v = e;
+      load_user(a, v, V);
+      if constexpr (per_voxel) {
+#pragma unroll
+        for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * a.N + v];
+      }
+    } else {
+      const long long cap = in.cap;
+      v = in.idx[e];
+#pragma unroll
+      for (int i = 0; i < M; ++i) V.y[i] = in.y[(long long)i * cap + e];
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        V.p[j] = in.p[(long long)j * cap + e];
+        V.lo[j] = in.lo[(long long)j * cap + e];
+        V.hi[j] = in.hi[(long long)j * cap + e];
+      }
+      V.lam = in.lam[e];
+      V.f = in.f[e];
+      V.f_first = in.f_first[e];
+      V.st = in.st[e];
+      cnt_total = in.cnt[e];
+      if constexpr (per_voxel) {
+#pragma unroll
+        for (int q = 0; q < NCV; ++q) cv[q] = in.cv[(long long)q * cap + e];
+      }
+    }
+  }
+  int cnt = 0;
+  if (valid) {
+#ifdef PCA_CONSTS_PER_VOXEL
+    auto cf = [&](int k, int i) { return cv[(k * M + i) % NCV]; };
+#else
+    const real* __restrict__ uc = J.uc;
+    auto cf = [&](int k, int i) { return uc[k * M + i]; };
+#endif
+    for (int j = 0; j < A.k && V.st != 0; ++j) {
+      iterate<Mdl, M, UNROLL>(V, S, cf);
+      if (A.it0 + j == 0) V.f_first = V.f;
+      ++cnt;
+    }
+  }
+  cnt_total += cnt;
+  const bool keep = valid && V.st != 0 && !A.last;
+  // one atomic per wavefront reserves the slots of its surviving voxels
+  const unsigned long long mask = __ballot(keep);
+  const int n_keep = __popcll(mask);
+  int base = 0;
+  if (lane_id() == 0 && n_keep > 0)
+    base = atomicAdd(&A.counts[(A.chunk + 1) * PCA_NMODELS + MI], n_keep);
+  base = __shfl(base, 0);
+  if (keep) {
+    const unsigned long long below = (lane_id() == 0) ? 0ull : (mask & (~0ull >> (64 - lane_id())));
+    const long long pos = base + __popcll(below);
+    const long long cap = out.cap;
+#pragma unroll
+    for (int i = 0; i < M; ++i) out.y[(long long)i * cap + pos] = V.y[i];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      out.p[(long long)j * cap + pos] = V.p[j];
+      out.lo[(long long)j * cap + pos] = V.lo[j];
+      out.hi[(long long)j * cap + pos] = V.hi[j];
+    }
+    out.lam[pos] = V.lam;
+    out.f[pos] = V.f;
+    out.f_first[pos] = V.f_first;
+    out.st[pos] = (signed char)V.st;
+    out.cnt[pos] = cnt_total;
+    out.idx[pos] = (int)v;
+    if constexpr (per_voxel) {
+#pragma unroll
+      for (int q = 0; q < NCV; ++q) out.cv[(long long)q * cap + pos] = cv[q];
+    }
+  } else if (valid) {
+    store_result(a, v, V, cnt_total, cnt > 0);
+  }
+}
+
+template <int MI>
+struct ModelAt;
+#define PCA_MODEL_AT(I)                                  \
+  template <>                                            \
+  struct ModelAt<I> {                                    \
+    typedef Model##I type;                               \
+    static constexpr int M = PCA_M##I;                   \
+    static constexpr int UNROLL = PCA_UNROLL_M##I;       \
+  };
+PCA_MODEL_AT(0)
+#if PCA_NMODELS > 1
+PCA_MODEL_AT(1)
+#endif
+#if PCA_NMODELS > 2
+PCA_MODEL_AT(2)
+#endif
+#if PCA_NMODELS > 3
+PCA_MODEL_AT(3)
+#endif
+
+template <int MI>
+__device__ __forceinline__ void dispatch(int m, const pca_chunk_args& A, long long e, bool valid) {
+  if constexpr (MI < PCA_NMODELS) {
+    if (m == MI) {
+      chunk_wave<MI, typename ModelAt<MI>::type, ModelAt<MI>::M, ModelAt<MI>::UNROLL>(A, e, valid);
+    } else {
+      dispatch<MI + 1>(m, A, e, valid);
+    }
+  }
+}
+
+}  // namespace pca
+
+// Grid-stride over 64-slot wave tasks; model m's queue occupies
+// ceil(count_m / 64) consecutive tasks, so every wavefront works on one model.
+extern "C" __global__ void __launch_bounds__(PCA_BLOCK) PCA_KATTR
+pca_lm_chunk(pca_chunk_args A) {
+  const int* cnt = A.counts + A.chunk * PCA_NMODELS;
+  long long tasks[PCA_NMODELS + 1];
+  tasks[0] = 0;
+#pragma unroll
+  for (int m = 0; m < PCA_NMODELS; ++m) tasks[m + 1] = tasks[m] + (cnt[m] + 63) / 64;
+  const long long total = tasks[PCA_NMODELS];
+  const long long waves_per_block = PCA_BLOCK / 64;
+  const long long nwaves = (long long)gridDim.x * waves_per_block;
+  for (long long t = (long long)blockIdx.x * waves_per_block + threadIdx.x / 64; t < total;
+       t += nwaves) {
+    int m = 0;
+#pragma unroll
+    for (int q = 1; q < PCA_NMODELS; ++q)
+      if (t >= tasks[q]) m = q;
+    const long long e = (t - tasks[m]) * 64 + pca::lane_id();
+    pca::dispatch<0>(m, A, e, e < cnt[m]);
+  }
+}
+
+// counts[0:NMODELS] = first[0:NMODELS], the rest 0 (no memset in a captured graph)
+struct pca_counts_init {
+  int first[8];
+};
+extern "C" __global__ void pca_reset_counts(int* counts, int n, pca_counts_init init) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    counts[i] = i < PCA_NMODELS ? init.first[i] : 0;
+}
+
+// The state setup() establishes (lsqnonlin.py:1036-1056): lambda = 5,
+// step_type = 1, plus f = 0 and n_iter = 0; a kernel so that a captured graph
+// can restart a fit without host work.
+extern "C" __global__ void pca_lm_reset_state(pca_lm_args a) {
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < a.N;
+       v += (long long)gridDim.x * blockDim.x) {
+    a.lam[v] = (real)5;
+    a.step_type[v] = 1;
+    a.f[v] = (real)0;
+    if (a.n_iter) a.n_iter[v] = 0;
   }
 }

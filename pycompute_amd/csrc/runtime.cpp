@@ -20,6 +20,9 @@ struct pca_module_s {
 struct pca_kernel_s {
   hipFunction_t fn;
 };
+struct pca_graph_s {
+  hipGraphExec_t exec;
+};
 
 namespace {
 
@@ -78,7 +81,7 @@ extern "C" {
 
 const char* pca_last_error(void) { return g_err.c_str(); }
 
-int32_t pca_abi_version(void) { return 1; }
+int32_t pca_abi_version(void) { return 2; }
 
 int pca_compile(const char* source, const char* program_name, const char* const* options,
                 int32_t n_options, void** code, size_t* size) {
@@ -191,6 +194,89 @@ int pca_launch(pca_kernel kernel, uint32_t gx, uint32_t gy, uint32_t gz, uint32_
     return fail(PCA_ERR_ARG, "pca_launch: bad launch shape");
   PCA_HIP(hipModuleLaunchKernel(kernel->fn, gx, gy, gz, bx, by, bz, shared_bytes,
                                 (hipStream_t)stream, params, nullptr));
+  return PCA_OK;
+}
+
+int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
+                          const pca_model_job* jobs, int32_t nmodels, int32_t* counts,
+                          const int32_t* chunk_iters, int32_t n_chunks, uint32_t grid,
+                          uint32_t block, void* stream) {
+  if (!chunk_kernel || !reset_kernel || !jobs || !counts || !chunk_iters)
+    return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: NULL argument");
+  if (nmodels < 1 || nmodels > 4) return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: 1..4 models");
+  if (n_chunks < 1 || grid == 0 || block == 0 || block > 1024 || block % 64)
+    return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: bad chunk count or launch shape");
+  for (int m = 0; m < nmodels; ++m) {
+    const pca_model_job& j = jobs[m];
+    if (j.a.N < 0 || j.a.N > 0x7fffffff) return fail(PCA_ERR_LIMIT, "voxels per model > 2^31-1");
+    for (int q = 0; q < 2; ++q)
+      if (j.a.N > 0 && (j.ws[q].cap < j.a.N || !j.ws[q].y || !j.ws[q].p || !j.ws[q].lo ||
+                        !j.ws[q].hi || !j.ws[q].lam || !j.ws[q].f || !j.ws[q].f_first ||
+                        !j.ws[q].idx || !j.ws[q].cnt || !j.ws[q].st))
+        return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: working set smaller than N or NULL");
+  }
+  for (int c = 0; c < n_chunks; ++c)
+    if (chunk_iters[c] < 1) return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: chunk of < 1 iteration");
+
+  // reset the queue counters: row 0 holds every model's voxel count
+  struct {
+    int32_t first[8];
+  } init;
+  std::memset(&init, 0, sizeof(init));
+  for (int m = 0; m < nmodels; ++m) init.first[m] = (int32_t)jobs[m].a.N;
+  int32_t total = (n_chunks + 1) * nmodels;
+  void* rp[3] = {&counts, &total, &init};
+  PCA_HIP(hipModuleLaunchKernel(reset_kernel->fn, (uint32_t)((total + 255) / 256), 1, 1, 256, 1,
+                                1, 0, (hipStream_t)stream, rp, nullptr));
+
+  // kernel argument block: jobs[nmodels], counts, chunk, it0, k, last
+  const size_t jb = sizeof(pca_model_job) * (size_t)nmodels;
+  std::vector<unsigned char> buf(jb + sizeof(int32_t*) + 4 * sizeof(int32_t));
+  std::memcpy(buf.data(), jobs, jb);
+  std::memcpy(buf.data() + jb, &counts, sizeof(counts));
+  int32_t* tail = (int32_t*)(buf.data() + jb + sizeof(int32_t*));
+  int32_t it0 = 0;
+  for (int c = 0; c < n_chunks; ++c) {
+    tail[0] = c;
+    tail[1] = it0;
+    tail[2] = chunk_iters[c];
+    tail[3] = (c == n_chunks - 1) ? 1 : 0;
+    void* kp[1] = {buf.data()};
+    PCA_HIP(hipModuleLaunchKernel(chunk_kernel->fn, grid, 1, 1, block, 1, 1, 0,
+                                  (hipStream_t)stream, kp, nullptr));
+    it0 += chunk_iters[c];
+  }
+  return PCA_OK;
+}
+
+int pca_graph_begin(void* stream) {
+  PCA_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return PCA_OK;
+}
+
+int pca_graph_end(void* stream, pca_graph* out) {
+  if (!out) return fail(PCA_ERR_ARG, "pca_graph_end: NULL out");
+  hipGraph_t g;
+  PCA_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+  hipGraphExec_t ge;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  *out = new pca_graph_s{ge};
+  return PCA_OK;
+}
+
+int pca_graph_launch(pca_graph graph, void* stream) {
+  if (!graph) return fail(PCA_ERR_ARG, "pca_graph_launch: NULL graph");
+  PCA_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+  return PCA_OK;
+}
+
+int pca_graph_destroy(pca_graph graph) {
+  if (!graph) return PCA_OK;
+  hipError_t e = hipGraphExecDestroy(graph->exec);
+  delete graph;
+  if (e != hipSuccess) return hip_fail(e, "hipGraphExecDestroy");
   return PCA_OK;
 }
 

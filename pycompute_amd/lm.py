@@ -45,6 +45,12 @@ class LMSpec:
     consts_per_voxel: bool = False
     fast_exp: bool = False
     strict: bool = False
+    # tuning knobs (pycompute_amd/autotune.py): measurement-loop unroll factor
+    # (0 = full for n <= 8, else 1), waves per SIMD hint (0 = compiler's
+    # choice), faster non-correctly-rounded FP32 divide/sqrt
+    unroll: int = 0
+    waves_per_eu: int = 0
+    fast_div: bool = False
 
     @property
     def nparam(self):
@@ -76,9 +82,14 @@ class LMProgram:
         self.dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
         self.funcid = spec.funcid()
         self.source = build_source(spec, self.dag)
+        if spec.strict and (spec.fast_exp or spec.fast_div):
+            raise ValueError("the strict build excludes fast_exp and fast_div")
         options = list(BASE_OPTIONS)
         if spec.strict:
             options.append("-ffp-contract=off")
+        if spec.fast_div:
+            options = [o for o in options if o != "-fhip-fp32-correctly-rounded-divide-sqrt"]
+            options.append("-fno-hip-fp32-correctly-rounded-divide-sqrt")
         self.module = Module(self.source, self.funcid + ("_strict" if spec.strict else ""),
                              options)
         self.flops = hipgen.flops_per_iteration(self.dag, spec.M)
@@ -96,7 +107,7 @@ def build_source(spec: LMSpec, dag) -> str:
     n = spec.nparam
     # Full unrolling of the measurement loop keeps consts in scalar registers;
     # large models keep the loop rolled to bound code size.
-    unroll = spec.M if n <= 8 else 1
+    unroll = spec.unroll if spec.unroll > 0 else (spec.M if n <= 8 else 1)
     head = [f"// generated: {spec.funcid()}",
             f"// model: {spec.expr}",
             f"// pars: {', '.join(spec.pars)}; consts: {', '.join(spec.consts)}; M = {spec.M}",
@@ -112,6 +123,8 @@ def build_source(spec: LMSpec, dag) -> str:
         head.append("#define PCA_FAST_EXP 1")
     if spec.strict:
         head.append("#define PCA_STRICT 1")
+    if spec.waves_per_eu > 0:
+        head.append(f"#define PCA_WAVES_PER_EU {spec.waves_per_eu}")
     return "\n".join([
         "\n".join(head),
         read_csrc("strict_math.h"),

@@ -31,6 +31,7 @@ import ctypes
 import torch
 
 from . import _lib
+from . import chunked
 from . import lm
 from .program import HipFunction
 from .sym import hipgen
@@ -137,7 +138,9 @@ class SecondOrderLevenbergMarquardt(HipFunction):
     """Batched second-order LM with the GMW81 step (lsqnonlin.py:949-1073)."""
 
     def __init__(self, *args, block: int = lm.DEFAULT_BLOCK, fast_exp: bool = False,
-                 keep_workspace: bool | None = None, strict: bool = False, **kwargs):
+                 keep_workspace: bool | None = None, strict: bool = False,
+                 compaction: bool = True, chunk: int = 8, unroll: int = 0,
+                 waves_per_eu: int = 0, fast_div: bool = False, **kwargs):
         if (args and isinstance(args[0], str)) or "expr" in kwargs:
             # notebook form: (expr, pars_str, consts_str, ndata, dtype, write_to_file)
             a = _bind(args, kwargs, ("expr", "pars_str", "consts_str", "ndata", "dtype",
@@ -179,11 +182,18 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         # strict: reproducible transcendentals and no FMA contraction, so the
         # CPU oracle replays the fit bit for bit
         self.strict = bool(strict)
+        # compaction: run the iterations in chunks of `chunk`, re-packing the
+        # unconverged voxels between chunks (identical results, fuller waves)
+        self.compaction = bool(compaction)
+        self.chunk = int(chunk)
+        self._ws = None
+        self._uc_t = None
+        self._chunk_state = None
+        self._knobs = dict(unroll=int(unroll), waves_per_eu=int(waves_per_eu),
+                           fast_div=bool(fast_div))
         self.keep_workspace = (self.nparam <= 8) if keep_workspace is None else bool(keep_workspace)
         if self.expr_true:
-            self._spec_uniform = lm.LMSpec(self.expr, tuple(self.pars_str), tuple(self.consts_str),
-                                           self.ndata, self.dtype_name, self.block, False,
-                                           self.fast_exp, self.strict)
+            self._spec_uniform = self._spec(False)
             self.funcid = self._spec_uniform.funcid()
         else:
             self.funcid = self._plugin.funcid
@@ -191,6 +201,11 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         self.run_func = None
         self.program = None
         self.build()
+
+    def _spec(self, consts_per_voxel: bool) -> lm.LMSpec:
+        return lm.LMSpec(self.expr, tuple(self.pars_str), tuple(self.consts_str), self.ndata,
+                         self.dtype_name, self.block, consts_per_voxel, self.fast_exp,
+                         self.strict, **self._knobs)
 
     # -- CudaFunction protocol ------------------------------------------------
     def get_device_funcid(self):
@@ -214,9 +229,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             self.program = self._plugin.program(consts_per_voxel)
             self.code = self.program.source
         else:
-            spec = lm.LMSpec(self.expr, tuple(self.pars_str), tuple(self.consts_str), self.ndata,
-                             self.dtype_name, self.block, consts_per_voxel, self.fast_exp,
-                             self.strict)
+            spec = self._spec(consts_per_voxel)
             try:
                 self.program = lm.get_program(spec)
             except _lib.PcaError:
@@ -272,6 +285,11 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         else:
             self.step_t = self.g_t = self.h_t = self.hl_t = None
         self.ftp_t = None
+        self._ws = None
+        self._uc_t = None
+        if self._chunk_state is not None:
+            self._chunk_state[1].destroy_graph()
+        self._chunk_state = None
         self.pars_tp_t = None
 
     def program_is_per_voxel(self) -> bool:
@@ -300,11 +318,78 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             return None
         if not self.expr_true:
             return self._plugin.run(self, iters, tol, mu, eta, acc, dec)
+        if self._use_chunked():
+            r = self._chunked_run(tol, mu, eta, acc, dec)
+            r.set_iters(iters)
+            r.launch(chunked.chunk_schedule(int(iters), self.chunk))
+            return None
         a = self.lm_args(iters, tol, mu, eta, acc, dec)
         k = self.program.kernel("pca_lm_fit")
         _lib.check(_lib.lib().pca_lm_launch(k, ctypes.byref(a), self._ucbuf, self._ucbytes,
                                             self.block, lm.stream_handle()), "pca_lm_launch")
         return None
+
+    # -- chunked execution with convergence compaction ------------------------
+    def _use_chunked(self) -> bool:
+        if not self.compaction:
+            return False
+        # per-voxel consts ride along in the queues only when they fit in registers
+        return self._cpv is None or self.nconst * self.ndata <= 64
+
+    def _chunked_run(self, tol, mu, eta, acc, dec):
+        key = (float(tol), float(mu), float(eta), float(acc), float(dec))
+        if self._chunk_state is not None and self._chunk_state[0] == key:
+            return self._chunk_state[1]
+        dev = self.data_t.device
+        if self._ws is None:
+            ncm = self.nconst * self.ndata if self._cpv is not None else 0
+            self._ws = chunked.Workset(self.Nprobs, self.ndata, self.nparam, ncm, self.dtype, dev)
+            if self._uc is not None:
+                self._uc_t = torch.tensor(self._uc, dtype=self.dtype, device=dev)
+        a = self.lm_args(0, tol, mu, eta, acc, dec)
+        r = chunked.ChunkedRun(self.program, [(self.program, a, self._uc_t, self._ws)], dev,
+                               self.block)
+        self._chunk_state = (key, r)
+        return r
+
+    def reset_state(self):
+        """Restore the setup() state on the device (lambda 5, step_type 1, f 0,
+        n_iter 0) with one kernel on the current stream."""
+        a = self.lm_args(0, 1e-8, 0.25, 0.75, 0.2, 2.0)
+        grid = max(1, min((self.Nprobs + 255) // 256, 4096))
+        params = (ctypes.c_void_p * 1)(ctypes.cast(ctypes.pointer(a), ctypes.c_void_p))
+        _lib.check(_lib.lib().pca_launch(self.program.kernel("pca_lm_reset_state"), grid, 1, 1,
+                                         256, 1, 1, 0, lm.stream_handle(), params),
+                   "pca_lm_reset_state")
+
+    def capture_graph(self, iters: int, tol=1e-8, mu=0.25, eta=0.75, acc=0.2, dec=2.0,
+                      init=None):
+        """Capture a whole fit -- reset_state(), optionally init(stream) (a
+        kernel writing the initial guess into pars_t), and the chunked run --
+        into a hipGraph; replay it with replay_graph()."""
+        if not self._use_chunked():
+            raise RuntimeError("graph capture needs the chunked path (compaction=True)")
+        r = self._chunked_run(tol, mu, eta, acc, dec)
+        r.set_iters(iters)
+        a = self.lm_args(0, tol, mu, eta, acc, dec)
+        self.program.kernel("pca_lm_reset_state")
+        L = _lib.lib()
+
+        def prologue(stream):
+            grid = max(1, min((self.Nprobs + 255) // 256, 4096))
+            params = (ctypes.c_void_p * 1)(ctypes.cast(ctypes.pointer(a), ctypes.c_void_p))
+            _lib.check(L.pca_launch(self.program.kernel("pca_lm_reset_state"), grid, 1, 1, 256,
+                                    1, 1, 0, stream, params), "pca_lm_reset_state")
+            if init is not None:
+                init(stream)
+
+        r.capture(chunked.chunk_schedule(int(iters), self.chunk), prologue)
+        return r
+
+    def replay_graph(self):
+        if self._chunk_state is None or self._chunk_state[1]._graph is None:
+            raise RuntimeError("capture_graph() first")
+        self._chunk_state[1].replay()
 
     # -- extras -----------------------------------------------------------------
     def final_f(self):
