@@ -211,6 +211,70 @@ int pca_loglinear_init(const double* P, int32_t K, int32_t M, const void* y, int
                        int32_t n, const void* lb, const void* ub, void* out, int64_t N,
                        int32_t is_double, void* stream);
 
+/* ---- learned initialiser (no counterpart in the reference; its initial
+ * guesses are the segmented log-linear fit and random search,
+ * test/ivim/test_ivim.ipynb cells 6 and 11-12). fp32 storage, bf16 MFMA
+ * operands (v_mfma_f32_16x16x16_bf16), fp32 accumulate. */
+
+/* C (M x N) = op(A) . op(B): op(A)[m][k] = transA ? A[k*lda+m] : A[m*lda+k],
+ * op(B)[k][n] = transB ? B[n*ldb+k] : B[k*ldb+n]. Epilogue epi: 0 store
+ * (with splits > 1: K split over blocks, added atomically into a zeroed C),
+ * 1 C = gelu(acc + bias[n]) and Z = acc + bias[n], 2 C = acc + bias[n],
+ * 3 C = acc * gelu'(Zin). */
+int pca_mlp_gemm(const float* A, int64_t lda, int32_t transA, const float* B, int64_t ldb,
+                 int32_t transB, float* C, int64_t ldc, int32_t M, int32_t N, int32_t K,
+                 int32_t epi, const float* bias, float* Z, const float* Zin, int32_t splits,
+                 void* stream);
+
+/* dY = 2 (Y - T) * scale; *loss += sum (Y - T)^2 * scale (loss may be NULL). */
+int pca_mlp_mse_grad(const float* Y, const float* T, float* dY, float* loss, int64_t n,
+                     float scale, void* stream);
+
+/* out[n] += sum_m X[m*ld + n] (bias gradients). */
+int pca_mlp_colsum(const float* X, int64_t ld, int32_t M, int32_t N, float* out, void* stream);
+
+/* One Adam step (bias-corrected, step >= 1) on a flat fp32 parameter vector. */
+int pca_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                 float b2, float eps, int32_t step, float weight_decay, void* stream);
+
+/* p[0:n] = value; fp32 -> bf16 (round to nearest even). */
+int pca_mlp_fill(float* p, int64_t n, float value, void* stream);
+int pca_mlp_to_bf16(const float* x, int16_t* y, int64_t n, void* stream);
+
+/* Training inputs: X (in_padded, N) = y / y[norm_index] (zero rows past in),
+ * T (N, out) = (transform(truth) - mu) / sd, the transform being: divide
+ * parameter scale_param by s = y[norm_index], subtract log s from
+ * logshift_param, take the log of the parameters in log_mask. */
+int pca_mlp_prepare(const float* y, const float* truth, int64_t N, int32_t in, int32_t in_padded,
+                    int32_t out, int32_t norm_index, int32_t scale_param, int32_t logshift_param,
+                    int32_t log_mask, const float* mu, const float* sd, float* X, float* T,
+                    void* stream);
+
+/* Fused inference of the in -> hidden -> hidden -> out GELU MLP: reads the
+ * signals (in, N), writes clip(inverse transform(mu + sd * y), lb, ub) into
+ * params (out, N) for voxels whose outputs are finite (others keep what
+ * params held: the fallback guess). w1 is (hidden, in_padded) with
+ * in_padded = 16, 32 or 64 and zero columns past `in`. */
+typedef struct pca_mlp_infer_args {
+  const void* data;      /* (in, N) fp32 */
+  const int16_t* w1;     /* (hidden, in_padded) bf16 */
+  const float* b1;       /* (hidden) */
+  const int16_t* w2;     /* (hidden, hidden) bf16 */
+  const float* b2;       /* (hidden) */
+  const int16_t* w3;     /* (out, hidden) bf16 */
+  const float* b3;       /* (out) */
+  const float* mu;       /* (out) */
+  const float* sd;       /* (out) */
+  const void* lb;        /* (out, N) fp32 */
+  const void* ub;        /* (out, N) fp32 */
+  void* params;          /* (out, N) fp32, in: fallback guess, out: MLP guess */
+  int64_t N;
+  int32_t in, out, hidden;
+  int32_t norm_index, scale_param, logshift_param, log_mask;
+  int32_t reserved;
+} pca_mlp_infer_args;
+int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream);
+
 /* gcnArchName of `device` (e.g. "gfx950:sramecc+:xnack-"). */
 int pca_device_arch(int32_t device, char* buf, int32_t len);
 

@@ -91,3 +91,94 @@ pca_eval(pca_eval_args a, pca_uconsts uc) {
   }
 }
 #undef PCA_ECONST
+
+// ---------------------------------------------------------------------------
+// Synthetic training signals from the model itself: parameters drawn
+// uniformly in [lo, hi] (or taken from truth_in), model evaluated at every
+// measurement, Rician noise y = |m + sigma (n1 + i n2)| with
+// sigma = m(measurement 0) / SNR and SNR uniform in [snr_lo, snr_hi].
+// Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC 2011) keyed by the seed, counter = (sample
+// index, draw index); normals by the Box-Muller transform.
+// ---------------------------------------------------------------------------
+
+struct pca_synth_args {
+  real* data;               // (M, N) out
+  real* truth;              // (NP, N) out (sampled or copied)
+  const real* truth_in;     // (NP, N) or null: use these parameters
+  const real* lo;           // (NP) sampling box
+  const real* hi;           // (NP)
+  double snr_lo, snr_hi;
+  unsigned long long seed;
+  long long N;
+  long long offset;         // sample index of entry 0 (counter base)
+};
+
+namespace pca {
+
+struct U4 {
+  unsigned x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, unsigned k0, unsigned k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// uniform in (0, 1)
+__device__ __forceinline__ double u01(unsigned a) { return ((double)a + 0.5) * 2.3283064365386963e-10; }
+
+}  // namespace pca
+
+extern "C" __global__ void __launch_bounds__(PCA_BLOCK)
+pca_synth(pca_synth_args a, pca_uconsts uc) {
+  constexpr int NP = Model::NP, NC = Model::NC;
+  const long long v = (long long)blockIdx.x * PCA_BLOCK + threadIdx.x;
+  if (v >= a.N) return;
+  const unsigned long long id = (unsigned long long)(a.offset + v);
+  const unsigned k0 = (unsigned)a.seed, k1 = (unsigned)(a.seed >> 32);
+  unsigned draw = 0;
+  auto next4 = [&]() {
+    pca::U4 c{(unsigned)id, (unsigned)(id >> 32), draw++, 0x5eedu};
+    return pca::philox4x32_10(c, k0, k1);
+  };
+  real p[NP];
+  if (a.truth_in) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) p[j] = a.truth_in[(long long)j * a.N + v];
+  } else {
+    for (int j = 0; j < NP; j += 4) {
+      const pca::U4 r = next4();
+      const unsigned rr[4] = {r.x, r.y, r.z, r.w};
+      for (int q = 0; q < 4 && j + q < NP; ++q)
+        p[j + q] = (real)((double)a.lo[j + q] + ((double)a.hi[j + q] - (double)a.lo[j + q]) * pca::u01(rr[q]));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) a.truth[(long long)j * a.N + v] = p[j];
+  const pca::U4 rs = next4();
+  const double snr = a.snr_lo + (a.snr_hi - a.snr_lo) * pca::u01(rs.x);
+  double sigma = 0.0;
+  for (int i = 0; i < PCA_M; ++i) {
+    real c[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) c[k] = uc.c[k * PCA_M + i];
+    const double m = (double)Model::resid(p, c, (real)0);
+    if (i == 0) sigma = m / snr;
+    const pca::U4 r = next4();
+    const double rad = sqrt(-2.0 * log(pca::u01(r.x)));
+    const double th = 6.283185307179586 * pca::u01(r.y);
+    const double n1 = rad * cos(th), n2 = rad * sin(th);
+    const double re = m + sigma * n1, im = sigma * n2;
+    a.data[(long long)i * a.N + v] = (real)sqrt(re * re + im * im);
+  }
+}

@@ -100,22 +100,31 @@ __device__ __forceinline__ float t_exp<float>(float a) { return expf(a); }
 template <>
 __device__ __forceinline__ double t_exp<double>(double a) { return exp(a); }
 
-template <typename T>
+// KC: compile-time size of the per-voxel arrays, max(rows of P, outputs)
+// (2, 3 or 4 for the built-in models; 0 = generic, up to kMaxK). The rows of
+// P actually read are Kr.
+template <typename T, int KC>
 __global__ void __launch_bounds__(256)
-k_loglinear_init(const double* __restrict__ P, int K, int M, const T* __restrict__ y, int kind,
+k_loglinear_init(const double* __restrict__ P, int Kr, int M, const T* __restrict__ y, int kind,
                  int n, const T* __restrict__ lb, const T* __restrict__ ub, T* __restrict__ out,
                  long long N) {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
-  double c[kMaxK];
-  for (int k = 0; k < K; ++k) c[k] = 0.0;
+  constexpr int KA = KC > 0 ? KC : kMaxK;
+  const int K = Kr;
+  double c[KA];
+#pragma unroll
+  for (int k = 0; k < KA; ++k) c[k] = 0.0;
   for (int i = 0; i < M; ++i) {
     const double ly = (double)t_log(y[(long long)i * N + v]);
-    for (int k = 0; k < K; ++k) c[k] += P[k * M + i] * ly;
+#pragma unroll
+    for (int k = 0; k < KA; ++k)
+      if (k < K) c[k] += P[k * M + i] * ly;
   }
-  T p[kMaxK];
+  T p[KA];
   if (kind == 0) {
-    for (int k = 0; k < K; ++k) p[k] = (T)c[k];
+#pragma unroll
+    for (int k = 0; k < KA; ++k) p[k] = (T)c[k];
   } else if (kind == 1) {
     p[0] = t_exp((T)c[0]);
     p[1] = (T)1 / (T)c[1];
@@ -131,12 +140,15 @@ k_loglinear_init(const double* __restrict__ P, int K, int M, const T* __restrict
     p[2] = (T)c[1];
     p[3] = (T)c[3];
   }
-  for (int j = 0; j < n; ++j) {
-    T a = nan_to_num(p[j]);
-    const T lo = lb[(long long)j * N + v], hi = ub[(long long)j * N + v];
-    a = a < lo ? lo : a;
-    a = a > hi ? hi : a;
-    out[(long long)j * N + v] = a;
+#pragma unroll
+  for (int j = 0; j < KA; ++j) {
+    if (j < n) {
+      T a = nan_to_num(p[j]);
+      const T lo = lb[(long long)j * N + v], hi = ub[(long long)j * N + v];
+      a = a < lo ? lo : a;
+      a = a > hi ? hi : a;
+      out[(long long)j * N + v] = a;
+    }
   }
 }
 
@@ -172,14 +184,24 @@ extern "C" int pca_loglinear_init(const double* P, int32_t K, int32_t M, const v
     return pca_fail(PCA_ERR_ARG, "pca_loglinear_init: K/n do not match the init kind");
   if (N == 0) return PCA_OK;
   const dim3 grid((unsigned)((N + 255) / 256)), block(256);
-  if (is_double)
-    hipLaunchKernelGGL(k_loglinear_init<double>, grid, block, 0, (hipStream_t)stream, P, K, M,
-                       (const double*)y, kind, n, (const double*)lb, (const double*)ub,
-                       (double*)out, (long long)N);
-  else
-    hipLaunchKernelGGL(k_loglinear_init<float>, grid, block, 0, (hipStream_t)stream, P, K, M,
-                       (const float*)y, kind, n, (const float*)lb, (const float*)ub,
-                       (float*)out, (long long)N);
+  hipStream_t st = (hipStream_t)stream;
+#define PCA_INIT(TT, KK)                                                                      \
+  hipLaunchKernelGGL((k_loglinear_init<TT, KK>), grid, block, 0, st, P, K, M, (const TT*)y,  \
+                     kind, n, (const TT*)lb, (const TT*)ub, (TT*)out, (long long)N)
+  // kind 2 (stretched exponential) writes 3 parameters from 2 rows of P
+  const int KS = kind == 2 ? 3 : K;
+  if (is_double) {
+    if (KS == 4) PCA_INIT(double, 4);
+    else if (KS == 3) PCA_INIT(double, 3);
+    else if (KS == 2) PCA_INIT(double, 2);
+    else PCA_INIT(double, 0);
+  } else {
+    if (KS == 4) PCA_INIT(float, 4);
+    else if (KS == 3) PCA_INIT(float, 3);
+    else if (KS == 2) PCA_INIT(float, 2);
+    else PCA_INIT(float, 0);
+  }
+#undef PCA_INIT
   {
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PCA_OK : pca_fail(PCA_ERR_HIP, hipGetErrorString(e));

@@ -1,0 +1,510 @@
+// Learned initialiser: a small MLP (in -> H -> H -> out, GELU) mapping a
+// voxel's normalised signal to standardised starting parameters. Its dense
+// contractions run on the matrix cores: v_mfma_f32_16x16x16_bf16 (bf16
+// operands, fp32 accumulate). Compiled into libpycompute_amd.so for gfx950.
+//
+// There is no counterpart in the reference (its initial guesses are the
+// segmented log-linear fit and random search, test/ivim/test_ivim.ipynb
+// cells 6, 11-12); LM refines from the MLP output and the log-linear guess is
+// the fallback.
+//
+// Storage is fp32 (weights keep fp32 masters for Adam); operands are rounded
+// to bf16 (round to nearest even) as they are loaded into MFMA registers.
+//
+// MFMA 16x16x16 bf16 operand layout (wave64, lane l):
+//   A (16 x 16, M x K): row l % 16, k = 4 (l / 16) + 0..3
+//   B (16 x 16, K x N): col l % 16, k = 4 (l / 16) + 0..3
+//   C (16 x 16):        col l % 16, rows 4 (l / 16) + 0..3
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "internal.h"
+#include "pycompute_amd.h"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ short f2bf(float x) { return __builtin_bit_cast(short, (__bf16)x); }
+
+__device__ __forceinline__ float bf2f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+
+__device__ __forceinline__ floatx4 mfma16(bf16x4 a, bf16x4 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
+// evaluated as x * sigmoid(2u) = x / (1 + exp(-2u)) with the hardware exp and
+// reciprocal (a few ulp; the result is rounded to bf16 next anyway).
+__device__ __forceinline__ float gelu(float x) {
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+}
+// d/dx x s(2u) = s + 2 x s (1 - s) u',  s = sigmoid(2u)
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  const float du = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+  return s + 2.0f * x * s * (1.0f - s) * du;
+}
+
+enum Epi { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_GRAD = 3 };
+
+struct GemmArgs {
+  const float* A;
+  const float* B;
+  float* C;
+  const float* bias;  // EPI_BIAS*: (N)
+  float* Z;           // EPI_BIAS_GELU: pre-activation out (M, ldc) or null
+  const float* Zin;   // EPI_GELU_GRAD: pre-activation in (M, ldc)
+  long long lda, ldb, ldc;
+  int M, N, K;
+  int transA, transB;
+  int epi;
+  int k_per_split;
+};
+
+// C (M x N) = op(A) (M x K) . op(B) (K x N), one 32 x 32 tile per wavefront
+// (2 x 2 MFMA tiles), 4 wavefronts per 64 x 64 block tile. gridDim.z > 1
+// splits K and adds partial tiles atomically (EPI_STORE only; C pre-zeroed).
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
+  const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+  const int kb = blockIdx.z * g.k_per_split;
+  const int ke = min(g.K, kb + g.k_per_split);
+  const int r = lane & 15;
+  const int kq = (lane >> 4) * 4;
+  floatx4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+    bf16x4 a[2], b[2];
+    for (int t = 0; t < 2; ++t) {
+      const int m = m0 + t * 16 + r;
+      const int n = n0 + t * 16 + r;
+      for (int q = 0; q < 4; ++q) {
+        const int k = k0 + kq + q;
+        float av = 0.f, bv = 0.f;
+        if (m < g.M && k < ke) av = g.transA ? g.A[(long long)k * g.lda + m] : g.A[(long long)m * g.lda + k];
+        if (n < g.N && k < ke) bv = g.transB ? g.B[(long long)n * g.ldb + k] : g.B[(long long)k * g.ldb + n];
+        a[t][q] = f2bf(av);
+        b[t][q] = f2bf(bv);
+      }
+    }
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+  }
+  const bool split = gridDim.z > 1;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + j * 16 + (lane & 15);
+      if (n >= g.N) continue;
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + i * 16 + (lane >> 4) * 4 + q;
+        if (m >= g.M) continue;
+        float v = acc[i][j][q];
+        const long long o = (long long)m * g.ldc + n;
+        if (split) {
+          atomicAdd(&g.C[o], v);
+          continue;
+        }
+        if (g.epi == EPI_BIAS_GELU) {
+          v += g.bias[n];
+          if (g.Z) g.Z[o] = v;
+          v = gelu(v);
+        } else if (g.epi == EPI_BIAS) {
+          v += g.bias[n];
+        } else if (g.epi == EPI_GELU_GRAD) {
+          v *= gelu_grad(g.Zin[o]);
+        }
+        g.C[o] = v;
+      }
+    }
+}
+
+// dY = 2 (Y - T) / (B * O); loss += sum (Y - T)^2 / (B * O)
+__global__ void k_mse_grad(const float* Y, const float* T, float* dY, float* loss, long long n,
+                           float scale) {
+  float acc = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float d = Y[i] - T[i];
+    dY[i] = 2.f * d * scale;
+    acc += d * d * scale;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, acc);
+}
+
+// out[n] += sum_m X[m * ld + n] (bias gradients), rows split over blocks
+__global__ void k_colsum(const float* X, long long ld, int M, int N, float* out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int rows = (M + gridDim.y - 1) / gridDim.y;
+  const int mb = blockIdx.y * rows, me = min(M, mb + rows);
+  float acc = 0.f;
+  for (int m = mb; m < me; ++m) acc += X[(long long)m * ld + n];
+  atomicAdd(&out[n], acc);
+}
+
+// Adam (Kingma and Ba 2015) with bias correction, on a flat parameter vector.
+__global__ void k_adam(float* w, const float* g, float* m, float* v, long long n, float lr,
+                       float b1, float b2, float eps, float c1, float c2, float wd) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float gi = g[i] + wd * w[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+__global__ void k_fill(float* p, long long n, float value) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+__global__ void k_to_bf16(const float* x, short* y, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = f2bf(x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: normalise, three layers, de-standardise, clip.
+// ---------------------------------------------------------------------------
+
+struct InferArgs {
+  const float* data;     // (IN, N) measurements, voxel fastest
+  const short* w1;       // (H, IN) bf16
+  const float* b1;       // (H)
+  const short* w2;       // (H, H) bf16
+  const float* b2;       // (H)
+  const short* w3;       // (OUT, H) bf16
+  const float* b3;       // (OUT)
+  const float* mu;       // (OUT) output de-standardisation: p = mu + sd * y
+  const float* sd;       // (OUT)
+  const float* lb;       // (OUT, N)
+  const float* ub;       // (OUT, N)
+  float* params;         // (OUT, N) out
+  long long N;
+  int IN, OUT;
+  int norm_index;        // measurement whose value s normalises the signal
+  int scale_param;       // parameter predicted relative to s (p *= s), -1: none
+  int logshift_param;    // parameter predicted as a log relative to s (p += log s), -1: none
+  int log_mask;          // bit j: parameter j is predicted as log(p_j)
+};
+
+// Register-resident inference in the transposed form: every layer computes
+// out^T (units x voxels) = W (units x k) . in^T (k x voxels). The MFMA result
+// layout (lane: column = voxel l%16, rows 4(l/16)+0..3 = units) is then
+// exactly the B-operand layout of the next layer (k = unit 4(l/16)+0..3,
+// column = voxel l%16), so activations stay in registers between layers, and
+// each weight fragment (A operand, 8 contiguous bytes per lane) is reused for
+// VT voxel tiles. Only the final OUT x 16VT outputs pass through LDS, to
+// gather each voxel's parameters into one lane.
+template <int H, int INP, int OUTP, int VT>
+__global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
+  constexpr int UT = H / 16;     // unit tiles per hidden layer
+  constexpr int KT1 = INP / 16;  // k-steps of layer 1
+  constexpr int OT = OUTP / 16;  // output tiles
+  constexpr int V = 16 * VT;     // voxels per wavefront
+  __shared__ float ybuf[4][OUTP * V];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int col = lane & 15;      // voxel within a tile
+  const int kq = (lane >> 4) * 4; // k (or row) offset within a tile
+  float* yb = ybuf[wave];
+  const long long nwaves = (long long)gridDim.x * 4;
+  for (long long base = ((long long)blockIdx.x * 4 + wave) * V; base < a.N; base += nwaves * V) {
+    // layer-1 B operands: normalised inputs x^T (k = feature, n = voxel)
+    bf16x4 xb[KT1][VT];
+#pragma unroll
+    for (int t = 0; t < VT; ++t) {
+      const long long v = base + t * 16 + col;
+      const bool ok = v < a.N;
+      float s = 1.f;
+      if (ok) {
+        s = a.data[(long long)a.norm_index * a.N + v];
+        if (!(s > 0.f)) s = 1.f;
+      }
+      const float inv = 1.f / s;
+#pragma unroll
+      for (int kt = 0; kt < KT1; ++kt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = kt * 16 + kq + q;
+          xb[kt][t][q] = f2bf((ok && i < a.IN) ? a.data[(long long)i * a.N + v] * inv : 0.f);
+        }
+    }
+    // layer 1: h1^T = gelu(W1 x^T + b1)
+    bf16x4 h1[UT][VT];
+#pragma unroll
+    for (int u = 0; u < UT; ++u) {
+      floatx4 acc[VT];
+#pragma unroll
+      for (int t = 0; t < VT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < KT1; ++kt) {
+        const bf16x4 w = *(const bf16x4*)(a.w1 + (long long)(u * 16 + col) * INP + kt * 16 + kq);
+#pragma unroll
+        for (int t = 0; t < VT; ++t) acc[t] = mfma16(w, xb[kt][t], acc[t]);
+      }
+#pragma unroll
+      for (int t = 0; t < VT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) h1[u][t][q] = f2bf(gelu(acc[t][q] + a.b1[u * 16 + kq + q]));
+    }
+    // layers 2 and 3 fused: each 16-unit tile of h2^T = gelu(W2 h1^T + b2) is
+    // consumed at once as one k-step of y^T = W3 h2^T + b3 (rows of W3 past
+    // OUT are zero), so h2 never needs to be held whole
+    floatx4 acc3[OT][VT];
+#pragma unroll
+    for (int o = 0; o < OT; ++o)
+#pragma unroll
+      for (int t = 0; t < VT; ++t) acc3[o][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // rolled over pairs of tiles: u only selects weights and biases, so only
+    // the weight fragments of two tiles are live at a time, and the two
+    // tiles' accumulators give the matrix pipe 2 VT independent chains
+#pragma unroll 1
+    for (int u0 = 0; u0 < UT; u0 += 2) {
+      floatx4 acc[2][VT];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < VT; ++t) acc[h][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < UT; ++kt) {
+        bf16x4 w[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          w[h] = *(const bf16x4*)(a.w2 + (long long)((u0 + h) * 16 + col) * H + kt * 16 + kq);
+#pragma unroll
+        for (int t = 0; t < VT; ++t)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) acc[h][t] = mfma16(w[h], h1[kt][t], acc[h][t]);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int u = u0 + h;
+        bf16x4 h2u[VT];
+#pragma unroll
+        for (int t = 0; t < VT; ++t)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            h2u[t][q] = f2bf(gelu(acc[h][t][q] + a.b2[u * 16 + kq + q]));
+#pragma unroll
+        for (int o = 0; o < OT; ++o) {
+          const int row = o * 16 + col;
+          bf16x4 w = bf16x4{0, 0, 0, 0};
+          if (row < a.OUT) w = *(const bf16x4*)(a.w3 + (long long)row * H + u * 16 + kq);
+#pragma unroll
+          for (int t = 0; t < VT; ++t) acc3[o][t] = mfma16(w, h2u[t], acc3[o][t]);
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < OT; ++o)
+#pragma unroll
+      for (int t = 0; t < VT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = o * 16 + kq + q;
+          yb[j * V + t * 16 + col] = acc3[o][t][q] + (j < a.OUT ? a.b3[j] : 0.f);
+        }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // one lane per voxel: inverse transform, finite check, clip, store
+    for (int vv = lane; vv < V; vv += 64) {
+      const long long v = base + vv;
+      if (v >= a.N) continue;
+      float s = a.data[(long long)a.norm_index * a.N + v];
+      if (!(s > 0.f)) s = 1.f;
+      bool finite = true;
+      float p[OUTP];
+#pragma unroll
+      for (int j = 0; j < OUTP; ++j) {
+        if (j < a.OUT) {
+          float t = a.mu[j] + a.sd[j] * yb[j * V + vv];
+          if ((a.log_mask >> j) & 1) t = expf(t);
+          if (j == a.scale_param) t *= s;
+          if (j == a.logshift_param) t += logf(s);
+          finite = finite && !(__builtin_isnan(t) || __builtin_isinf(t));
+          p[j] = t;
+        }
+      }
+      if (finite) {
+#pragma unroll
+        for (int j = 0; j < OUTP; ++j) {
+          if (j < a.OUT) {
+            const float lo = a.lb[(long long)j * a.N + v], hi = a.ub[(long long)j * a.N + v];
+            float t = p[j] < lo ? lo : p[j];
+            t = t > hi ? hi : t;
+            a.params[(long long)j * a.N + v] = t;
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Training inputs from signals and true parameters: X (INP, N) = y / y[norm]
+// (rows >= IN zero), T (N, OUT) = (transform(p) - mu) / sd with the transform
+// that k_mlp_infer_t inverts.
+__global__ void k_prepare(const float* y, const float* truth, long long N, int IN, int INP,
+                          int OUT, int norm_index, int scale_param, int logshift_param,
+                          int log_mask, const float* mu, const float* sd, float* X, float* T) {
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+       v += (long long)gridDim.x * blockDim.x) {
+    float s = y[(long long)norm_index * N + v];
+    if (!(s > 0.f)) s = 1.f;
+    for (int i = 0; i < INP; ++i) X[(long long)i * N + v] = i < IN ? y[(long long)i * N + v] / s : 0.f;
+    for (int j = 0; j < OUT; ++j) {
+      float t = truth[(long long)j * N + v];
+      if (j == scale_param) t /= s;
+      if (j == logshift_param) t -= logf(s);
+      if ((log_mask >> j) & 1) t = logf(t);
+      T[v * OUT + j] = (t - mu[j]) / sd[j];
+    }
+  }
+}
+
+int grid_1d(long long n) { return (int)std::min<long long>((n + 255) / 256, 4096); }
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PCA_OK : pca_fail(PCA_ERR_HIP, hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" int pca_mlp_gemm(const float* A, int64_t lda, int32_t transA, const float* B,
+                            int64_t ldb, int32_t transB, float* C, int64_t ldc, int32_t M,
+                            int32_t N, int32_t K, int32_t epi, const float* bias, float* Z,
+                            const float* Zin, int32_t splits, void* stream) {
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0 || epi < 0 || epi > 3 || splits < 1)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm: bad argument");
+  if ((epi == EPI_BIAS_GELU || epi == EPI_BIAS) && !bias)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm: epilogue needs a bias");
+  if (epi == EPI_GELU_GRAD && !Zin) return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm: needs Zin");
+  if (splits > 1 && epi != EPI_STORE)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm: split-K only with the plain epilogue");
+  GemmArgs g{A, B, C, bias, Z, Zin, lda, ldb, ldc, M, N, K, transA, transB, epi, 0};
+  int kps = (K + splits - 1) / splits;
+  kps = (kps + 15) / 16 * 16;
+  g.k_per_split = kps;
+  const int zs = (K + kps - 1) / kps;
+  dim3 grid((M + 63) / 64, (N + 63) / 64, zs);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, (hipStream_t)stream, g);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_mse_grad(const float* Y, const float* T, float* dY, float* loss, int64_t n,
+                                float scale, void* stream) {
+  if (!Y || !T || !dY || n <= 0) return pca_fail(PCA_ERR_ARG, "pca_mlp_mse_grad: bad argument");
+  hipLaunchKernelGGL(k_mse_grad, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, Y, T, dY,
+                     loss, (long long)n, scale);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_colsum(const float* X, int64_t ld, int32_t M, int32_t N, float* out,
+                              void* stream) {
+  if (!X || !out || M <= 0 || N <= 0) return pca_fail(PCA_ERR_ARG, "pca_mlp_colsum: bad argument");
+  dim3 grid((N + 255) / 256, std::min(256, (M + 127) / 128));
+  hipLaunchKernelGGL(k_colsum, grid, dim3(256), 0, (hipStream_t)stream, X, (long long)ld, M, N,
+                     out);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr,
+                            float b1, float b2, float eps, int32_t step, float weight_decay,
+                            void* stream) {
+  if (!w || !g || !m || !v || n <= 0 || step < 1)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_adam: bad argument");
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v,
+                     (long long)n, lr, b1, b2, eps, c1, c2, weight_decay);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_fill(float* p, int64_t n, float value, void* stream) {
+  if (!p || n < 0) return pca_fail(PCA_ERR_ARG, "pca_mlp_fill: bad argument");
+  if (n == 0) return PCA_OK;
+  hipLaunchKernelGGL(k_fill, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, p,
+                     (long long)n, value);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_to_bf16(const float* x, int16_t* y, int64_t n, void* stream) {
+  if (!x || !y || n <= 0) return pca_fail(PCA_ERR_ARG, "pca_mlp_to_bf16: bad argument");
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, x,
+                     (short*)y, (long long)n);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
+  if (!args || !args->data || !args->w1 || !args->w2 || !args->w3 || !args->params ||
+      !args->lb || !args->ub || !args->mu || !args->sd || !args->b1 || !args->b2 || !args->b3)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: NULL argument");
+  const int H = args->hidden, IN = args->in, OUT = args->out;
+  if (IN < 1 || IN > 64 || OUT < 1 || OUT > 32 || (H != 128 && H != 256) || args->N < 0 ||
+      args->norm_index < 0 || args->norm_index >= IN)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: sizes must be in <= 64, out <= 32, hidden 128|256");
+  if (args->N == 0) return PCA_OK;
+  InferArgs a{(const float*)args->data, (const short*)args->w1, args->b1, (const short*)args->w2,
+              args->b2, (const short*)args->w3, args->b3, args->mu, args->sd,
+              (const float*)args->lb, (const float*)args->ub, (float*)args->params, args->N, IN,
+              OUT, args->norm_index, args->scale_param, args->logshift_param, args->log_mask};
+  // the input width is padded to a multiple of 16 and stored with that stride
+  const int INP = IN <= 16 ? 16 : (IN <= 32 ? 32 : 64);
+  const int OUTP = OUT <= 16 ? 16 : 32;
+  const bool big = H == 256;
+  const int V = big ? 32 : 64;  // voxels per wavefront
+  const long long waves = (args->N + V - 1) / V;
+  const int grid = (int)std::min<long long>((waves + 3) / 4, 8192);
+  hipStream_t s = (hipStream_t)stream;
+#define PCA_INFER(HH, II, OO, VT) \
+  hipLaunchKernelGGL((k_mlp_infer_t<HH, II, OO, VT>), dim3(grid), dim3(256), 0, s, a)
+  if (!big) {
+    if (INP == 16 && OUTP == 16) PCA_INFER(128, 16, 16, 4);
+    else if (INP == 16) PCA_INFER(128, 16, 32, 4);
+    else if (INP == 32 && OUTP == 16) PCA_INFER(128, 32, 16, 4);
+    else if (INP == 32) PCA_INFER(128, 32, 32, 4);
+    else if (OUTP == 16) PCA_INFER(128, 64, 16, 4);
+    else PCA_INFER(128, 64, 32, 4);
+  } else {
+    if (INP == 16 && OUTP == 16) PCA_INFER(256, 16, 16, 2);
+    else if (INP == 16) PCA_INFER(256, 16, 32, 2);
+    else if (INP == 32 && OUTP == 16) PCA_INFER(256, 32, 16, 2);
+    else if (INP == 32) PCA_INFER(256, 32, 32, 2);
+    else if (OUTP == 16) PCA_INFER(256, 64, 16, 2);
+    else PCA_INFER(256, 64, 32, 2);
+  }
+#undef PCA_INFER
+  return check_launch();
+}
+
+extern "C" int pca_mlp_prepare(const float* y, const float* truth, int64_t N, int32_t in,
+                               int32_t in_padded, int32_t out, int32_t norm_index,
+                               int32_t scale_param, int32_t logshift_param, int32_t log_mask,
+                               const float* mu, const float* sd, float* X, float* T,
+                               void* stream) {
+  if (!y || !truth || !mu || !sd || !X || !T || N <= 0 || in < 1 || in_padded < in ||
+      out < 1 || out > 32 || norm_index < 0 || norm_index >= in)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_prepare: bad argument");
+  hipLaunchKernelGGL(k_prepare, dim3(grid_1d(N)), dim3(256), 0, (hipStream_t)stream, y, truth,
+                     (long long)N, in, in_padded, out, norm_index, scale_param, logshift_param,
+                     log_mask, mu, sd, X, T);
+  return check_launch();
+}

@@ -1,0 +1,169 @@
+"""GPU checks of the learned initialiser: the bf16-MFMA GEMM against torch,
+training gradients against the CPU oracle (oracle/mlp.py, bf16 rounding at the
+same points), fused inference against the oracle forward, the synthetic
+signal generator against the model, and what the trained network gains."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as omlp
+from pycompute_amd import fit, synth
+from pycompute_amd import mlp
+from pycompute_amd.models import IVIM
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+@pytest.mark.parametrize("transA,transB", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 36, 70), (4, 128, 1000), (513, 4, 128)])
+def test_gemm_matches_torch(gpu, transA, transB, M, N, K):
+    g = torch.Generator(device="cuda").manual_seed(M * 7 + N + K)
+    A = torch.randn((K, M) if transA else (M, K), device="cuda", generator=g)
+    B = torch.randn((N, K) if transB else (K, N), device="cuda", generator=g)
+    ref = (_bf(A).T if transA else _bf(A)) @ (_bf(B).T if transB else _bf(B))
+    for splits in (1, 3):
+        C = torch.zeros((M, N), device="cuda")
+        mlp.gemm(A, A.shape[1], transA, B, B.shape[1], transB, C, N, M, N, K, splits=splits)
+        torch.cuda.synchronize()
+        err = (C - ref).abs().max().item() / ref.abs().max().item()
+        assert err < 1e-5, (splits, err)
+
+
+def test_gemm_epilogues(gpu):
+    g = torch.Generator(device="cuda").manual_seed(3)
+    M, N, K = 130, 48, 40
+    A = torch.randn((M, K), device="cuda", generator=g)
+    B = torch.randn((K, N), device="cuda", generator=g)
+    bias = torch.randn(N, device="cuda", generator=g)
+    acc = _bf(A) @ _bf(B)
+    C = torch.empty((M, N), device="cuda")
+    Z = torch.empty((M, N), device="cuda")
+    mlp.gemm(A, K, 0, B, N, 0, C, N, M, N, K, mlp.EPI_BIAS_GELU, bias, Z)
+    torch.cuda.synchronize()
+    z = (acc + bias).double().cpu().numpy()
+    np.testing.assert_allclose(Z.cpu().numpy(), z, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(C.cpu().numpy(), omlp.gelu(z), rtol=1e-4, atol=1e-5)
+    C2 = torch.empty((M, N), device="cuda")
+    mlp.gemm(A, K, 0, B, N, 0, C2, N, M, N, K, mlp.EPI_GELU_GRAD, Zin=Z)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(C2.cpu().numpy(), acc.double().cpu().numpy() * omlp.gelu_grad(z),
+                               rtol=1e-4, atol=1e-4)
+
+
+def _small_net(seed=0):
+    net = mlp.ivim_mlp(synth.B_C2, seed=seed)
+    return net
+
+
+def test_training_gradients_match_oracle(gpu):
+    net = _small_net()
+    rng = np.random.default_rng(0)
+    B = 512
+    X = rng.uniform(0.1, 1.2, (net.INP, B)).astype(np.float32)
+    X[net.IN:] = 0
+    T = rng.standard_normal((B, net.OUT)).astype(np.float32)
+    Xt = torch.tensor(X, device="cuda")
+    Tt = torch.tensor(T, device="cuda")
+    net._buffers(B)["loss"].zero_()
+    net.forward_backward(Xt, Tt, 0, B, B)
+    torch.cuda.synchronize()
+    w = {k: net._view(net.w, k).double().cpu().numpy() for k in net.offsets}
+    loss, g, _ = omlp.forward_backward(X.T.astype(np.float64), T.astype(np.float64), w["w1"],
+                                       w["b1"], w["w2"], w["b2"], w["w3"], w["b3"])
+    assert abs(net._buf["loss"].item() - loss) / loss < 1e-4
+    for k in net.offsets:
+        got = net._view(net.g, k).double().cpu().numpy()
+        rel = np.linalg.norm(got - g[k]) / np.linalg.norm(g[k])
+        print(f"grad {k}: relative error {rel:.2e}", flush=True)
+        assert rel < 2e-3, (k, rel)
+
+
+def test_fused_inference_matches_oracle_forward(gpu):
+    net = _small_net(seed=1)
+    w = synth.ivim_c2(3000, seed=9)
+    prob = fit.make_problem("ivim", w["consts"], w["data"])
+    net.mu.copy_(torch.tensor([0.05, 0.2, -3.5, -6.5]))
+    net.sd.copy_(torch.tensor([0.1, 0.1, 0.5, 0.5]))
+    net.finalize()
+    params = torch.zeros((4, prob.N), device="cuda")
+    lb = torch.full_like(params, -1e30)
+    ub = torch.full_like(params, 1e30)
+    net.infer(prob.data_t, lb, ub, params)
+    torch.cuda.synchronize()
+    y = w["data"].astype(np.float64)
+    s = y[0]
+    X = np.zeros((net.INP, prob.N))
+    X[:net.IN] = y / s
+    wt = {k: net._view(net.w, k).double().cpu().numpy() for k in net.offsets}
+    out = omlp.forward(X.T, wt["w1"], wt["b1"], wt["w2"], wt["b2"], wt["w3"], wt["b3"])
+    mu, sd = net.mu.double().cpu().numpy(), net.sd.double().cpu().numpy()
+    p = mu + sd * out
+    p[:, 2:] = np.exp(p[:, 2:])
+    p[:, 0] *= s
+    got = params.double().cpu().numpy().T
+    rel = np.abs(got - p) / np.abs(p)
+    print("inference rel err p99", np.percentile(rel, 99), "max", rel.max(), flush=True)
+    assert np.percentile(rel, 99) < 2e-2 and np.median(rel) < 2e-3
+
+
+def test_synth_matches_model(gpu):
+    from pycompute_amd.lsqnonlin import Expr, model_eval
+    net = _small_net()
+    truth = torch.tensor(synth.ivim_truth(4096, np.random.default_rng(3)), dtype=torch.float32,
+                         device="cuda")
+    data, tru = net.synthesize(4096, seed=5, snr=(1e12, 1e12), truth=truth)
+    m, _, _ = model_eval(Expr(IVIM.expr, IVIM.pars, IVIM.consts), truth,
+                         torch.tensor(synth.B_C2, dtype=torch.float32, device="cuda"), 16,
+                         want=("m",))
+    torch.cuda.synchronize()
+    assert torch.equal(tru, truth)
+    np.testing.assert_allclose(data.cpu().numpy(), m.cpu().numpy(), rtol=1e-5)
+    # sampled parameters stay inside the box; noise has the requested level
+    lo, hi = [0.8, 0.05, 5e-3, 0.5e-3], [1.2, 0.4, 0.1, 3e-3]
+    d2, t2 = net.synthesize(200000, seed=6, snr=(50.0, 50.0), lo=lo, hi=hi)
+    t2 = t2.cpu().numpy()
+    for j in range(4):
+        assert t2[j].min() >= lo[j] and t2[j].max() <= hi[j]
+    m2, _, _ = model_eval(Expr(IVIM.expr, IVIM.pars, IVIM.consts), torch.tensor(t2, device="cuda"),
+                          torch.tensor(synth.B_C2, dtype=torch.float32, device="cuda"), 16,
+                          want=("m",))
+    # at b = 800 the signal is well above sigma: residual std ~ sigma = S0 / 50
+    r = (d2[-1] - m2[-1]) / torch.tensor(t2[0], device="cuda") * 50.0
+    assert abs(float(r.std()) - 1.0) < 0.05
+
+
+def test_trained_network_gains_over_default_init(gpu):
+    """Trained on 2M forward-model signals, the MLP guess lets LM reach the
+    total final SSR of plain LM (default guess, 100 iterations) with far fewer
+    iterations."""
+    net = _small_net(seed=2)
+    lo, hi = [0.8, 0.05, 5e-3, 0.5e-3], [1.2, 0.4, 0.1, 3e-3]
+    data, truth = net.synthesize(1 << 21, seed=1234, snr=(20.0, 100.0), lo=lo, hi=hi)
+    net.fit_standardization(truth, data)
+    X, T = net.prepare(data, truth)
+    hist = net.train(X, T, epochs=2, batch=8192)
+    print("loss history", hist, flush=True)
+    # f and D_1 are poorly determined by noisy signals, so the standardised
+    # loss stays well above 0; what matters is the LM gain below
+    assert hist[-1] < hist[0]
+    w = synth.ivim(65536, synth.B_C2, seed=77, snr=(20.0, 100.0))
+    prob = fit.make_problem("ivim", w["consts"], w["data"])
+    plain = fit.fit(prob, iters=100)
+    f_plain = plain.final_f().double().sum().item()
+    it_plain = plain.n_iter_t.double().mean().item()
+    p0 = fit.default_init(prob)
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, p0)
+    res = {}
+    for cap in (10, 15, 20):
+        s = fit.fit(prob, iters=cap, params0=p0.clone())
+        res[cap] = (s.final_f().double().sum().item() / f_plain - 1.0,
+                    s.n_iter_t.double().mean().item())
+    torch.cuda.synchronize()
+    print(f"plain: total SSR {f_plain:.4f}, {it_plain:.1f} iterations/voxel; mlp: {res}", flush=True)
+    best = [c for c, (d, it) in res.items() if d <= 1e-4]
+    assert best, res
+    assert res[min(best)][1] * 2.0 < it_plain
