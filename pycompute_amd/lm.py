@@ -131,6 +131,7 @@ def build_source(spec: LMSpec, dag) -> str:
         read_csrc("prelude.hip"),
         hipgen.emit_model(dag, sname, spec.dtype),
         f"typedef {sname} Model;",
+        read_csrc("lm_solver.hip"),
         read_csrc("lm_kernel.hip"),
         read_csrc("eval_kernel.hip"),
     ])

@@ -262,7 +262,13 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         self.lower_bound_t = _as_rows(lower_bound_t, self.nparam, N, "lower_bound_t")
         self.upper_bound_t = _as_rows(upper_bound_t, self.nparam, N, "upper_bound_t")
         self.consts_t = consts_t
-        self._uc, self._cpv = split_consts(consts_t, self.nconst, self.ndata, N, uniform_consts)
+        if not self.expr_true:
+            # plugin code indexes consts itself: passed through as given
+            self._uc, self._cpv = None, consts_t.contiguous()
+            self.consts_t = self._cpv
+        else:
+            self._uc, self._cpv = split_consts(consts_t, self.nconst, self.ndata, N,
+                                               uniform_consts)
         if (self._cpv is not None) != self.program_is_per_voxel():
             self.build(consts_per_voxel=self._cpv is not None)
         if self._uc is not None:

@@ -158,3 +158,44 @@ class Module:
                            f"kernel {kname} of {self.name}")
                 self._kernels[key] = kh
                 return kh
+
+
+# ---------------------------------------------------------------------------
+# host helpers for packed lower-triangle storage (l = i(i+1)/2 + j, j <= i),
+# cuda_program.py:208-235
+# ---------------------------------------------------------------------------
+
+def compact_to_full(mat):
+    """Packed (n(n+1)/2,) lower triangle -> full symmetric (n, n)."""
+    import math
+    import numpy as np
+    m = np.asarray(mat.cpu() if hasattr(mat, "cpu") else mat)
+    n = int(math.floor(math.sqrt(2 * m.shape[0])))
+    out = np.empty((n, n), dtype=np.float64)
+    k = 0
+    for i in range(n):
+        for j in range(i + 1):
+            out[i, j] = out[j, i] = m[k]
+            k += 1
+    return out
+
+
+def compact_to_LD(mat):
+    """Packed L\\D factor (unit-lower L below the diagonal, D on it, as GMW81
+    leaves it) -> (L, D) as full matrices."""
+    import math
+    import numpy as np
+    m = np.asarray(mat.cpu() if hasattr(mat, "cpu") else mat)
+    n = int(math.floor(math.sqrt(2 * m.shape[0])))
+    L = np.zeros((n, n))
+    D = np.zeros((n, n))
+    k = 0
+    for i in range(n):
+        for j in range(i + 1):
+            if i != j:
+                L[i, j] = m[k]
+            else:
+                L[i, j] = 1.0
+                D[i, j] = m[k]
+            k += 1
+    return L, D

@@ -132,3 +132,40 @@ def test_synthetic_workloads_are_seeded():
     assert d["data"].shape == (61, 100) and d["consts"].shape == (4, 61)
     g = d["consts"][1:, 1:]
     np.testing.assert_allclose(np.linalg.norm(g, axis=0), 1.0, rtol=1e-12)
+
+
+def test_packed_matrix_helpers():
+    from pycompute_amd.program import compact_to_LD, compact_to_full
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((5, 5))
+    A = A + A.T
+    packed = np.array([A[i, j] for i in range(5) for j in range(i + 1)])
+    np.testing.assert_array_equal(compact_to_full(packed), A)
+    L, D = compact_to_LD(packed)
+    assert np.allclose(np.diag(L), 1.0) and np.allclose(np.triu(L, 1), 0.0)
+    np.testing.assert_array_equal(np.diag(D), np.diag(A))
+    np.testing.assert_array_equal(np.tril(L, -1), np.tril(A, -1))
+
+
+def test_code_gen_walking_links_post_order_once():
+    from pycompute_amd.program import HipFunction, code_gen_walking
+
+    class Fn(HipFunction):
+        def __init__(self, fid, deps=()):
+            self.fid, self.deps = fid, list(deps)
+
+        def get_device_funcid(self):
+            return self.fid
+
+        def get_device_code(self):
+            return f"/*{self.fid}*/"
+
+        def get_deps(self):
+            return self.deps
+
+    a = Fn("a")
+    b = Fn("b", [a])
+    c = Fn("c", [a, b])
+    code = code_gen_walking(Fn("top", [b, c]))
+    order = [s for s in code.split() if s.startswith("/*")]
+    assert order == ["/*a*/", "/*b*/", "/*c*/", "/*top*/"]
