@@ -10,6 +10,7 @@ of the caller's own problem:
                    per-voxel data arrays in scratch, so full wins for small n)
   * fast_exp / fast_div: v_exp_f32 and the approximate FP32 divide (only when
                    the caller allows non-reference rounding)
+  * waves_per_eu:  occupancy hint (mixed-model batches, batch_candidates())
 
 The fastest candidate whose results agree with the reference-rounding build on
 the sample (final SSR within 1e-4 relative for at least as many voxels as the
@@ -65,12 +66,28 @@ def candidates(nparam: int, allow_fast: bool):
     return out
 
 
+def batch_candidates(allow_fast: bool = True):
+    """Knobs for a mixed-model batch: one kernel serves every model, so its
+    register allocation is the largest model's; the occupancy hint (waves per
+    SIMD) trades that against spills."""
+    out = []
+    fast = [(False, False)] + ([(True, True)] if allow_fast else [])
+    for fe, fd in fast:
+        for chunk in (8, 16):
+            for wpe in (0, 2):
+                out.append({"block": 256, "chunk": chunk, "unroll": 0, "fast_exp": fe,
+                            "fast_div": fd, "waves_per_eu": wpe})
+    return out
+
+
 def tune(make_solver, setup, model_expr: str, M: int, dtype: str, allow_fast: bool = True,
-         iters: int = 100, reps: int = 3, force: bool = False, log=None) -> dict:
+         iters: int = 100, reps: int = 3, force: bool = False, log=None,
+         cands=None) -> dict:
     """Return the best knob set for this problem, timing candidates if needed.
 
     make_solver(**knobs) -> solver;  setup(solver) binds the sample tensors and
-    restores the initial guess (called before every timed run).
+    restores the initial guess (called before every timed run). cands: the
+    knob sets to time (default: candidates()).
     """
     arch = torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName
     k = key(model_expr, M, dtype, allow_fast, arch)
@@ -79,7 +96,7 @@ def tune(make_solver, setup, model_expr: str, M: int, dtype: str, allow_fast: bo
         return dict(db[k]["best"])
     results = []
     ref_f = None
-    for cand in candidates(0, allow_fast):
+    for cand in (cands or candidates(0, allow_fast)):
         s = make_solver(compaction=True, **cand)
         ts = []
         for r in range(reps + 1):

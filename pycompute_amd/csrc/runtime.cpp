@@ -260,8 +260,12 @@ int pca_graph_end(void* stream, pca_graph* out) {
   PCA_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
   hipGraphExec_t ge;
   hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  const hipError_t ed = hipGraphDestroy(g);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  if (ed != hipSuccess) {
+    (void)hipGraphExecDestroy(ge);
+    return hip_fail(ed, "hipGraphDestroy");
+  }
   *out = new pca_graph_s{ge};
   return PCA_OK;
 }

@@ -72,16 +72,27 @@ class LMSpec:
 
 
 class LMProgram:
-    def __init__(self, spec: LMSpec):
-        if spec.M <= 0:
+    """One module. ``members`` (LMSpecs with voxel-uniform consts) are further
+    models compiled into the same pca_lm_chunk kernel for batched launches."""
+
+    def __init__(self, spec: LMSpec, members: tuple = ()):
+        if spec.M <= 0 or any(m.M <= 0 for m in members):
             raise ValueError("ndata must be positive")
         if spec.nconst * spec.M > _lib.PCA_UC_MAX and not spec.consts_per_voxel:
             raise ValueError(f"nconst*M = {spec.nconst * spec.M} exceeds {_lib.PCA_UC_MAX}; "
                              "use per-voxel consts")
+        if len(members) > 3:
+            raise ValueError("at most 4 models share one module")
         self.spec = spec
+        self.members = tuple(members)
         self.dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
+        self.member_dags = [hipgen.res_jac_hes(m.expr, m.pars, m.consts) for m in members]
         self.funcid = spec.funcid()
-        self.source = build_source(spec, self.dag)
+        if members:
+            ids = "|".join(m.funcid() for m in (spec,) + self.members)
+            self.funcid = f"batched_levenberg_marquardt_{len(ids.split('|'))}_" \
+                          f"{hipgen.expr_hash(ids, 12)}"
+        self.source = build_source(spec, self.dag, list(zip(self.members, self.member_dags)))
         if spec.strict and (spec.fast_exp or spec.fast_div):
             raise ValueError("the strict build excludes fast_exp and fast_div")
         options = list(BASE_OPTIONS)
@@ -93,6 +104,8 @@ class LMProgram:
         self.module = Module(self.source, self.funcid + ("_strict" if spec.strict else ""),
                              options)
         self.flops = hipgen.flops_per_iteration(self.dag, spec.M)
+        self.member_flops = [hipgen.flops_per_iteration(d, m.M)
+                             for m, d in zip(self.members, self.member_dags)]
 
     @property
     def nhes(self):
@@ -102,19 +115,29 @@ class LMProgram:
         return self.module.kernel(name)
 
 
-def build_source(spec: LMSpec, dag) -> str:
-    sname = "PcaModel_" + hipgen.expr_hash(spec.expr, 12)
-    n = spec.nparam
+def _unroll(spec: LMSpec) -> int:
     # Full unrolling of the measurement loop keeps consts in scalar registers;
     # large models keep the loop rolled to bound code size.
-    unroll = spec.unroll if spec.unroll > 0 else (spec.M if n <= 8 else 1)
-    head = [f"// generated: {spec.funcid()}",
-            f"// model: {spec.expr}",
-            f"// pars: {', '.join(spec.pars)}; consts: {', '.join(spec.consts)}; M = {spec.M}",
-            f"#define PCA_M {spec.M}",
-            f"#define PCA_BLOCK {spec.block}",
-            f"#define PCA_UNROLL_M {unroll}",
-            f"#define PCA_UC_MAX {_lib.PCA_UC_MAX}"]
+    return spec.unroll if spec.unroll > 0 else (spec.M if spec.nparam <= 8 else 1)
+
+
+def build_source(spec: LMSpec, dag, members=()) -> str:
+    """HIP source of one module. ``members``: further (LMSpec, dag) pairs that
+    share the module's pca_lm_chunk kernel as Model1, Model2, ... (the
+    launch/code-generation knobs of ``spec`` apply to all of them)."""
+    models = [(spec, dag)] + list(members)
+    head = [f"// generated: {spec.funcid()}"]
+    for i, (s, _) in enumerate(models):
+        head += [f"// model {i}: {s.expr}",
+                 f"// pars: {', '.join(s.pars)}; consts: {', '.join(s.consts)}; M = {s.M}"]
+    head += [f"#define PCA_M {spec.M}",
+             f"#define PCA_BLOCK {spec.block}",
+             f"#define PCA_UNROLL_M {_unroll(spec)}",
+             f"#define PCA_UC_MAX {_lib.PCA_UC_MAX}"]
+    if members:
+        head.append(f"#define PCA_NMODELS {len(models)}")
+        for i, (s, _) in enumerate(models):
+            head += [f"#define PCA_M{i} {s.M}", f"#define PCA_UNROLL_M{i} {_unroll(s)}"]
     if spec.dtype == "float64":
         head.append("#define PCA_REAL_DOUBLE 1")
     if spec.consts_per_voxel:
@@ -125,12 +148,21 @@ def build_source(spec: LMSpec, dag) -> str:
         head.append("#define PCA_STRICT 1")
     if spec.waves_per_eu > 0:
         head.append(f"#define PCA_WAVES_PER_EU {spec.waves_per_eu}")
+    structs, typedefs = {}, []
+    for i, (s, d) in enumerate(models):
+        sname = "PcaModel_" + hipgen.expr_hash(s.expr, 12)
+        if sname not in structs:
+            structs[sname] = hipgen.emit_model(d, sname, spec.dtype)
+        if members:
+            typedefs.append(f"typedef {sname} Model{i};")
+        if i == 0:
+            typedefs.append(f"typedef {sname} Model;")
     return "\n".join([
         "\n".join(head),
         read_csrc("strict_math.h"),
         read_csrc("prelude.hip"),
-        hipgen.emit_model(dag, sname, spec.dtype),
-        f"typedef {sname} Model;",
+        "\n".join(structs.values()),
+        "\n".join(typedefs),
         read_csrc("lm_solver.hip"),
         read_csrc("lm_kernel.hip"),
         read_csrc("eval_kernel.hip"),
@@ -141,12 +173,13 @@ _programs: dict = {}
 _plock = threading.Lock()
 
 
-def get_program(spec: LMSpec) -> LMProgram:
+def get_program(spec: LMSpec, members: tuple = ()) -> LMProgram:
+    key = (spec, tuple(members))
     with _plock:
-        p = _programs.get(spec)
+        p = _programs.get(key)
         if p is None:
-            p = LMProgram(spec)
-            _programs[spec] = p
+            p = LMProgram(spec, tuple(members))
+            _programs[key] = p
         return p
 
 

@@ -361,12 +361,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
     def reset_state(self):
         """Restore the setup() state on the device (lambda 5, step_type 1, f 0,
         n_iter 0) with one kernel on the current stream."""
-        a = self.lm_args(0, 1e-8, 0.25, 0.75, 0.2, 2.0)
-        grid = max(1, min((self.Nprobs + 255) // 256, 4096))
-        params = (ctypes.c_void_p * 1)(ctypes.cast(ctypes.pointer(a), ctypes.c_void_p))
-        _lib.check(_lib.lib().pca_launch(self.program.kernel("pca_lm_reset_state"), grid, 1, 1,
-                                         256, 1, 1, 0, lm.stream_handle(), params),
-                   "pca_lm_reset_state")
+        launch_reset_state(self.program, self.lm_args(0, 1e-8, 0.25, 0.75, 0.2, 2.0),
+                           self.Nprobs, lm.stream_handle())
 
     def capture_graph(self, iters: int, tol=1e-8, mu=0.25, eta=0.75, acc=0.2, dec=2.0,
                       init=None):
@@ -379,13 +375,9 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         r.set_iters(iters)
         a = self.lm_args(0, tol, mu, eta, acc, dec)
         self.program.kernel("pca_lm_reset_state")
-        L = _lib.lib()
 
         def prologue(stream):
-            grid = max(1, min((self.Nprobs + 255) // 256, 4096))
-            params = (ctypes.c_void_p * 1)(ctypes.cast(ctypes.pointer(a), ctypes.c_void_p))
-            _lib.check(L.pca_launch(self.program.kernel("pca_lm_reset_state"), grid, 1, 1, 256,
-                                    1, 1, 0, stream, params), "pca_lm_reset_state")
+            launch_reset_state(self.program, a, self.Nprobs, stream)
             if init is not None:
                 init(stream)
 
@@ -404,6 +396,15 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         F.launch(self.program, self.pars_t, self.data_t, self._cpv, self._ucbuf, self._ucbytes,
                  None, out, self.block)
         return out
+
+
+def launch_reset_state(program, args, N, stream):
+    """pca_lm_reset_state over N voxels: lambda 5, step_type 1, f 0, n_iter 0
+    (the state setup() allocates, lsqnonlin.py:1048-1056)."""
+    grid = max(1, min((int(N) + 255) // 256, 4096))
+    params = (ctypes.c_void_p * 1)(ctypes.cast(ctypes.pointer(args), ctypes.c_void_p))
+    _lib.check(_lib.lib().pca_launch(program.kernel("pca_lm_reset_state"), grid, 1, 1, 256, 1,
+                                     1, 0, stream, params), "pca_lm_reset_state")
 
 
 class F(HipFunction):

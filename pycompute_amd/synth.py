@@ -126,3 +126,11 @@ def dki(N, seed=6, snr=30.0):
     S = np.exp(A @ truth)             # (61, N)
     y = _rician(S, np.exp(truth[0]) / snr, rng)
     return {"consts": consts, "data": y.astype(np.float32), "truth": truth}
+
+
+def mixed(n_t2=400_000, n_ivim=400_000, n_se=200_000, seed=7, snr=40.0):
+    """Mixed-model workload: mono-exponential T2 (32 echoes), IVIM (16
+    b-values 0-800) and stretched exponential (16 b-values), Rician SNR 40,
+    each with its own seeded stream."""
+    return {"t2": t2(n_t2, seed, snr), "ivim": ivim(n_ivim, B_C2, seed + 1, snr),
+            "se": se(n_se, seed + 2, snr)}
