@@ -1,5 +1,9 @@
 """GPU checks of the remaining reference API surface: the F_GradF plugin hook
-(user device code with the reference signatures) and bpolyfit."""
+(user device code with the reference signatures), bpolyfit, the standalone
+GMW81 solver, and the committed golden fixtures."""
+import json
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -122,3 +126,58 @@ def test_bpolyfit_matches_least_squares(gpu):
     torch.cuda.synchronize()
     np.testing.assert_allclose(out.cpu().numpy(), np.stack([np.full(N, 2.0), np.full(N, -3.0)]),
                                atol=1e-10)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("n", [4, 22])
+@pytest.mark.parametrize("dt,tag", [(torch.float32, "f32"), (torch.float64, "f64")])
+def test_gmw81_solver_bit_exact_on_stress_fixture(gpu, n, dt, tag):
+    """The reference's GMW81 stress recipe (test/linalg/test_gmw81.py:17-37):
+    the strict build reproduces the oracle's solutions bit for bit."""
+    from pycompute_amd.solver import GMW81Solver
+    g = np.load(os.path.join(GOLDEN, "gmw81_stress.npz"))
+    mat = torch.tensor(g[f"mat{n}"], dtype=dt, device="cuda")
+    rhs = torch.tensor(g[f"rhs{n}"], dtype=dt, device="cuda")
+    sol = torch.empty_like(rhs)
+    GMW81Solver(n, dt, strict=True).run(mat, rhs, sol)
+    torch.cuda.synchronize()
+    exp = g[f"sol{n}_{tag}"]
+    got = sol.cpu().numpy()
+    print(n, tag, "bit-exact systems", float(np.mean(np.all(got == exp, axis=0))), flush=True)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_gmw81_solver_reference_kat(gpu):
+    from pycompute_amd.solver import GMW81Solver
+    with open(os.path.join(GOLDEN, "kat_test_mats.json")) as fh:
+        d = json.load(fh)
+    A = np.array(d["A"])
+    packed = np.array([A[i, j] for i in range(4) for j in range(i + 1)])
+    for dt, rtol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        mat = torch.tensor(packed[:, None].repeat(3, 1), dtype=dt, device="cuda")
+        rhs = torch.tensor(np.array(d["b"])[:, None].repeat(3, 1), dtype=dt, device="cuda")
+        sol = torch.empty_like(rhs)
+        GMW81Solver(4, dt).run(mat, rhs, sol)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(sol.cpu().numpy(), np.array(d["x"])[:, None].repeat(3, 1),
+                                   rtol=rtol)
+
+
+def test_ivim_c1_golden_fixture(gpu):
+    """The default GPU build on the committed C1 fixture, started from the
+    fixture's initial guess: final SSR within 1e-4 of the oracle's FP32 result
+    (as close as the FP32 oracle is to its FP64 result)."""
+    d = np.load(os.path.join(GOLDEN, "ivim_c1_256.npz"))
+    prob = fit.make_problem("ivim", d["consts"], d["data"])
+    p0 = torch.tensor(d["p0"], dtype=torch.float32, device="cuda")
+    s = fit.fit(prob, iters=100, params0=p0)
+    f = s.final_f().double().cpu().numpy()[0]
+    rel32 = np.abs(f - d["ssr_f32"]) / d["ssr_f32"]
+    rel64 = np.abs(f - d["ssr_f64"]) / d["ssr_f64"]
+    o = np.abs(d["ssr_f32"] - d["ssr_f64"]) / d["ssr_f64"]
+    print("C1 fixture: vs f32", np.mean(rel32 <= 1e-4), "vs f64", np.mean(rel64 <= 1e-4),
+          "oracle f32 vs f64", np.mean(o <= 1e-4), flush=True)
+    assert np.mean(rel32 <= 1e-4) >= 0.97
+    assert np.mean(rel64 <= 1e-4) >= np.mean(o <= 1e-4) - 0.02
