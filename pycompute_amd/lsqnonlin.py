@@ -140,7 +140,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
     def __init__(self, *args, block: int = lm.DEFAULT_BLOCK, fast_exp: bool = False,
                  keep_workspace: bool | None = None, strict: bool = False,
                  compaction: bool = True, chunk: int = 8, unroll: int = 0,
-                 waves_per_eu: int = 0, fast_div: bool = False, **kwargs):
+                 waves_per_eu: int = 0, fast_div: bool = False, wide: bool | None = None,
+                 tile: int = 136, **kwargs):
         if (args and isinstance(args[0], str)) or "expr" in kwargs:
             # notebook form: (expr, pars_str, consts_str, ndata, dtype, write_to_file)
             a = _bind(args, kwargs, ("expr", "pars_str", "consts_str", "ndata", "dtype",
@@ -192,9 +193,20 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         self._knobs = dict(unroll=int(unroll), waves_per_eu=int(waves_per_eu),
                            fast_div=bool(fast_div))
         self.keep_workspace = (self.nparam <= 8) if keep_workspace is None else bool(keep_workspace)
+        # wide: the kernel for models with more parameters than one lane's
+        # registers hold (csrc/lm_glm.hip; generalised linear models with
+        # voxel-uniform consts, n > 8). None = use it whenever it applies.
+        self.wide, self.tile = wide, int(tile)
+        self._glm, self._glm_tab = None, None
         if self.expr_true:
             self._spec_uniform = self._spec(False)
             self.funcid = self._spec_uniform.funcid()
+            if wide is not False:
+                from . import wide as _wide
+                self._glm = _wide.glm_program(self._spec_uniform, self.tile)
+                if wide and self._glm is None:
+                    raise ValueError("the wide kernel needs n > 8, a non-strict build and a "
+                                     "model phi(eta) with eta affine in the parameters")
         else:
             self.funcid = self._plugin.funcid
         self.Nprobs = 1
@@ -275,6 +287,9 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             self._ucbuf, self._ucbytes = lm.uconsts_buffer(self._uc, self.dtype_name)
         else:
             self._ucbuf, self._ucbytes = None, 0
+        self._glm_tab = None
+        if self._glm is not None and self._uc is not None:
+            self._glm_tab = self._glm.table(self._uc, self.data_t.device)
         dev = self.data_t.device
         kw = dict(dtype=self.dtype, device=dev)
         self.lam_t = torch.full((1, N), 5.0, **kw)
@@ -324,6 +339,9 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             return None
         if not self.expr_true:
             return self._plugin.run(self, iters, tol, mu, eta, acc, dec)
+        if self._glm_tab is not None:
+            self._glm.launch(self.lm_args(iters, tol, mu, eta, acc, dec), self._glm_tab)
+            return None
         if self._use_chunked():
             r = self._chunked_run(tol, mu, eta, acc, dec)
             r.set_iters(iters)

@@ -52,10 +52,14 @@ _DKI_W = ("gx**4", "gy**4", "gz**4", "4*gx**3*gy", "4*gx**3*gz", "4*gx*gy**3", "
           "12*gx**2*gy*gz", "12*gx*gy**2*gz", "12*gx*gy*gz**2")
 DKI_EXPR = ("exp(lnS0 - b*(" + " + ".join(f"{a}*{d}" for a, d in zip(_DKI_A, DKI_D)) + ")"
             + " + b*b/6*(" + " + ".join(f"{w}*{k}" for w, k in zip(_DKI_W, DKI_K)) + "))")
+# Bounds: physical ranges with margin. Diffusivities up to 5e-3 mm^2/s; the
+# K terms MD^2 W stay below (3e-3)^2 * ~2.2 = 2e-5 for kurtosis up to 2, so
+# +-2e-4 keeps exp(eta) finite at b = 2000 (b^2/6 * 2e-4 = 133 per unit w)
+# while leaving a 10x margin.
 DKI = ModelSpec(
     "dki", DKI_EXPR, ("lnS0",) + DKI_D + DKI_K, ("b", "gx", "gy", "gz"),
-    (-10.0,) + (0.0, 0.0, 0.0, -5e-3, -5e-3, -5e-3) + (-0.1,) * 15,
-    (10.0,) + (5e-3, 5e-3, 5e-3, 5e-3, 5e-3, 5e-3) + (0.1,) * 15)
+    (-10.0,) + (0.0, 0.0, 0.0, -5e-3, -5e-3, -5e-3) + (-2e-4,) * 15,
+    (10.0,) + (5e-3, 5e-3, 5e-3, 5e-3, 5e-3, 5e-3) + (2e-4,) * 15)
 
 MODELS = {m.name: m for m in (IVIM, T2, SE, DKI)}
 

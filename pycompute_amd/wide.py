@@ -1,0 +1,96 @@
+"""Programs of the wide-model kernel (csrc/lm_glm.hip) for generalised linear
+models with more parameters than one lane's registers hold (n > 8)."""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+import torch
+
+from . import _lib, lm
+from .program import BASE_OPTIONS, Module, read_csrc
+from .sym import glm, hipgen
+
+WIDE_MIN_PARAMS = 9
+DEFAULT_TILE = 136
+
+
+class GLMProgram:
+    def __init__(self, spec: lm.LMSpec, tile_max: int = DEFAULT_TILE):
+        self.spec = spec
+        self.dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
+        self.form = glm.glm_form(self.dag)
+        if self.form is None:
+            raise ValueError("the wide kernel needs a model phi(eta) with eta affine in the "
+                             "parameters")
+        self.tile_max = int(tile_max)
+        n = spec.nparam
+        self.stride = glm.table_stride(n)
+        self.funcid = spec.funcid() + f"_glm{self.tile_max}"
+        head = [f"// generated: {self.funcid}", f"// model: {spec.expr}",
+                f"#define PCA_NP {n}", f"#define PCA_M {spec.M}", f"#define PCA_TS {self.stride}",
+                f"#define PCA_TILE_MAX {self.tile_max}", f"#define PCA_BLOCK {spec.block}"]
+        if spec.dtype == "float64":
+            head.append("#define PCA_REAL_DOUBLE 1")
+        if spec.fast_exp:
+            head.append("#define PCA_FAST_EXP 1")
+        if self.form.phi == "identity":
+            head.append("#define PCA_GLM_IDENTITY 1")
+        self.source = "\n".join(["\n".join(head), read_csrc("strict_math.h"),
+                                 read_csrc("prelude.hip"), read_csrc("lm_solver.hip"),
+                                 read_csrc("lm_glm.hip")])
+        options = list(BASE_OPTIONS)
+        if spec.fast_div:
+            options = [o for o in options if o != "-fhip-fp32-correctly-rounded-divide-sqrt"]
+            options.append("-fno-hip-fp32-correctly-rounded-divide-sqrt")
+        self.module = Module(self.source, self.funcid, options)
+        fl = hipgen.flops_per_iteration(self.dag, spec.M)
+        self.flops = glm_flops(n, spec.M, fl)
+
+    def table(self, uc, device) -> torch.Tensor:
+        """Coefficient table for voxel-uniform consts (k*M+i ordering)."""
+        c = np.asarray(uc, np.float64).reshape(self.spec.nconst, self.spec.M)
+        tab = glm.coefficient_table(self.form, self.dag, c)
+        return torch.tensor(tab, dtype=lm.torch_dtype(self.spec.dtype), device=device)
+
+    def launch(self, args: _lib.LMArgs, tab: torch.Tensor, stream=None):
+        t = ctypes.c_void_p(tab.data_ptr())
+        params = (ctypes.c_void_p * 2)(ctypes.cast(ctypes.pointer(args), ctypes.c_void_p),
+                                       ctypes.cast(ctypes.pointer(t), ctypes.c_void_p))
+        grid = (int(args.N) + 63) // 64
+        s = lm.stream_handle() if stream is None else stream
+        _lib.check(_lib.lib().pca_launch(self.module.kernel("pca_lm_glm"), grid, 1, 1, 64, 1, 1,
+                                         0, s, params), "pca_lm_glm")
+
+
+def glm_flops(n: int, M: int, generic: dict) -> dict:
+    """Algorithmic FLOPs of one iteration of the GLM formulation (the
+    parameter-dependent work; the consts-only table is evaluated once):
+    per measurement eta (2n), r, w, f (6), g and damping (2n + 3n + 3),
+    w A_j (n) and the packed update (2 n(n+1)/2); the trial pass (2n + 3 per
+    measurement); the solve and step as in the generic count."""
+    nh = n * (n + 1) // 2
+    per_meas = 2 * n + 6 + 5 * n + 3 + n + 2 * nh
+    trial = 2 * n + 3
+    f_it = M * (per_meas + trial) + generic["F_solve"] + generic["F_step"]
+    f_conv = M * per_meas + generic["F_solve"] + 4 * n + 2
+    return {"F_it": f_it, "F_conv": f_conv, "exp_it": 2 * M, "exp_conv": M,
+            "F_solve": generic["F_solve"], "F_step": generic["F_step"]}
+
+
+_programs: dict = {}
+_lock = threading.Lock()
+
+
+def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> GLMProgram | None:
+    """The wide-kernel program for ``spec``, or None when the model is not a
+    GLM or small enough for the register kernel."""
+    if spec.nparam < WIDE_MIN_PARAMS or spec.strict:
+        return None
+    key = (spec, tile_max)
+    with _lock:
+        if key not in _programs:
+            dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
+            _programs[key] = GLMProgram(spec, tile_max) if glm.glm_form(dag) else None
+        return _programs[key]

@@ -1,0 +1,46 @@
+"""Host side of the wide-model kernel: GLM structure detection and the
+coefficient table (no GPU)."""
+import numpy as np
+import pytest
+
+from pycompute_amd import synth
+from pycompute_amd.models import DKI, IVIM, SE, T2, dki_design
+from pycompute_amd.sym import glm, hipgen
+
+
+def test_dki_is_exp_glm_and_table_is_the_design_matrix():
+    dag = hipgen.res_jac_hes(DKI.expr, DKI.pars, DKI.consts)
+    form = glm.glm_form(dag)
+    assert form is not None and form.phi == "exp" and len(form.coef) == 22
+    consts = synth.dki_protocol()
+    tab = glm.coefficient_table(form, dag, consts)
+    n = 22
+    assert tab.shape == (61, glm.table_stride(n)) and glm.table_stride(n) == 48
+    # the closed-form design matrix of log S (models.dki_design)
+    np.testing.assert_allclose(tab[:, :n], dki_design(consts), rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(tab[:, n:2 * n], tab[:, :n] ** 2)
+    np.testing.assert_array_equal(tab[:, 2 * n], 0.0)
+    np.testing.assert_array_equal(tab[:, 2 * n + 1:], 0.0)
+
+
+@pytest.mark.parametrize("m", [IVIM, T2, SE])
+def test_nonlinear_models_are_not_glm(m):
+    assert glm.glm_form(hipgen.res_jac_hes(m.expr, m.pars, m.consts)) is None
+
+
+def test_identity_glm_with_offset():
+    expr = "3*t + a + b*cos(t) + c*sin(2*t)"
+    dag = hipgen.res_jac_hes(expr, ["a", "b", "c"], ["t"])
+    form = glm.glm_form(dag)
+    assert form.phi == "identity"
+    t = np.linspace(0, 6, 7)
+    tab = glm.coefficient_table(form, dag, t[None, :])
+    np.testing.assert_allclose(tab[:, 0], 1.0)
+    np.testing.assert_allclose(tab[:, 1], np.cos(t), rtol=1e-12)
+    np.testing.assert_allclose(tab[:, 2], np.sin(2 * t), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(tab[:, 6], 3 * t, rtol=1e-12)       # offset a0 = eta(p = 0)
+
+
+def test_exp_of_nonlinear_exponent_is_rejected():
+    dag = hipgen.res_jac_hes("exp(a*b*t)", ["a", "b"], ["t"])
+    assert glm.glm_form(dag) is None
