@@ -80,6 +80,19 @@ IVIM_BOX = ([0.8, 0.05, 5e-3, 0.5e-3], [1.2, 0.4, 0.1, 3e-3])   # training param
 ITER_CAPS = (2, 3, 4, 5, 6, 8, 10, 12, 15, 20, 25, 30, 40, 50, 70, 100)
 
 
+def _time_replays(replay, args):
+    """ms per call of replay() over args.steps calls after args.warmup."""
+    import torch
+    for _ in range(args.warmup):
+        replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        replay()
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0) / args.steps
+
+
 def run_mlp(args, prob, knobs, plain_ssr, stream):
     """Train the initialiser on forward-model signals (separate seed), choose
     the LM iteration cap on a held-out validation volume (the smallest cap whose
@@ -121,6 +134,26 @@ def run_mlp(args, prob, knobs, plain_ssr, stream):
             cap = c
             break
     log(f"validation: iteration cap {cap}")
+    # what capping alone buys: the default guess at its own validated cap
+    pdef = fit.default_init(pv)
+    dcap = ITER_CAPS[-1]
+    for c in ITER_CAPS:
+        s = fit.fit(pv, iters=c, params0=pdef.clone(), **knobs)
+        if float(s.final_f().double().sum()) <= ref_ssr * (1 + 1e-4):
+            dcap = c
+            break
+    dsolver = fit.make_solver(prob, compaction=True, **knobs)
+    dpars = torch.empty((4, N), dtype=torch.float32, device=prob.data_t.device)
+    dsolver.setup(dpars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+    dsolver.capture_graph(dcap, init=lambda st: fit._init.default_init(
+        prob.model, prob.consts, prob.data_t, prob.lb_t, prob.ub_t, out=dpars, stream=st))
+    dms = _time_replays(dsolver.replay_graph, args)
+    dssr = float(dsolver.final_f().double().sum())
+    default_capped = {"iter_cap": dcap, "ms_per_step": dms, "value": N / (dms * 1e-3),
+                      "total_final_ssr": dssr,
+                      "matched_total_residual": bool(dssr <= plain_ssr * (1 + 1e-4))}
+    log(f"default guess: validated cap {dcap}, {dms:.3f} ms/step")
+    del dsolver
 
     # the benchmark volume, one whole MLP-initialised fit per step
     solver = fit.make_solver(prob, compaction=True, **knobs)
@@ -182,7 +215,9 @@ def run_mlp(args, prob, knobs, plain_ssr, stream):
             "lm_kernel_ms": lm_ms,
             "lm_tflops": lm_flops / (lm_ms * 1e-3) / 1e12,
             "lm_frac_of_fp32_peak": lm_flops / (lm_ms * 1e-3) / PEAK_FP32,
-            "validation": {"voxels": pv.N, "seed": 99, "plain_total_ssr": ref_ssr}}
+            "validation": {"voxels": pv.N, "seed": 99, "plain_total_ssr": ref_ssr},
+            "default_init_capped": default_capped,
+            "_net": net, "_cap": cap}
 
 
 def run_mixed(args, stream):
@@ -276,6 +311,171 @@ def run_mixed(args, stream):
             "lm_frac_of_fp32_peak": alg / (k_ms * 1e-3) / PEAK_FP32}
 
 
+def run_ivim_4m(args, knobs, net, cap):
+    """The 4,194,304-voxel IVIM volume (16 b-values, SNR U[20, 100], seed 4):
+    plain LM (default guess, at most 100 iterations) and the MLP-initialised
+    fit at the cap validated on the 1M run, both as captured graphs."""
+    import torch
+    from pycompute_amd import fit, synth
+
+    w = synth.ivim(1 << 22, synth.B_C2, seed=4, snr=(20.0, 100.0))
+    prob = fit.make_problem("ivim", w["consts"], w["data"])
+    N = prob.N
+    out = {"workload": "IVIM 4,194,304 voxels, 16 b-values 0-800, SNR U[20,100]",
+           "voxels": N}
+    for name, iters, use_mlp in (("plain_lm", 100, False), ("mlp_lm", cap, True)):
+        s = fit.make_solver(prob, compaction=True, **knobs)
+        pars = torch.empty((4, N), dtype=torch.float32, device=prob.data_t.device)
+        s.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+
+        def init(st, pars=pars, use_mlp=use_mlp):
+            fit._init.default_init(prob.model, prob.consts, prob.data_t, prob.lb_t, prob.ub_t,
+                                   out=pars, stream=st)
+            if use_mlp:
+                net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, stream=st)
+
+        s.capture_graph(iters, init=init)
+        ms = _time_replays(s.replay_graph, args)
+        out[name] = {"iter_cap": iters, "ms_per_step": ms, "value": N / (ms * 1e-3),
+                     "total_final_ssr": float(s.final_f().double().sum()),
+                     "iterations_per_voxel": float(s.n_iter_t.double().mean())}
+        del s
+    out["speedup"] = out["plain_lm"]["ms_per_step"] / out["mlp_lm"]["ms_per_step"]
+    out["total_ssr_rel_vs_plain"] = (out["mlp_lm"]["total_final_ssr"] /
+                                     out["plain_lm"]["total_final_ssr"] - 1.0)
+    log(f"IVIM 4M: plain {out['plain_lm']['ms_per_step']:.3f} ms, MLP-init "
+        f"{out['mlp_lm']['ms_per_step']:.3f} ms, SSR {out['total_ssr_rel_vs_plain']:+.2e}")
+    return out
+
+
+def run_dki(args, stream):
+    """22-parameter kurtosis model, 2,097,152 voxels (61 measurements: b=0 and
+    30 directions at b=1000 and 2000, SNR 30, seed 6), wide-model kernel.
+    Plain LM from the default (log-linear) guess for at most 100 iterations
+    against the MLP (61 -> 256 -> 256 -> 22) guess, guarded per voxel by the
+    default guess, at the iteration cap validated on a held-out volume; the
+    default guess at its own validated cap is reported beside it. Residual
+    totals leave out the voxels plain LM itself fails on (final SSR non-finite
+    or above 100x the median)."""
+    import torch
+    from pycompute_amd import fit, mlp, synth
+    from pycompute_amd import init as _init
+
+    rr = args.reference_rounding
+    kn = {"fast_exp": not rr, "fast_div": not rr}
+    w = synth.dki(1 << 21, seed=6)
+    prob = fit.make_problem("dki", w["consts"], w["data"])
+    N = prob.N
+    p0 = fit.default_init(prob)
+    solver = fit.make_solver(prob, **kn)
+    pars = torch.empty((22, N), dtype=torch.float32, device=prob.data_t.device)
+    solver.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+    fl = solver._glm.flops
+
+    def fit_step(iters, guess):
+        guess(pars)
+        solver.reset_state()
+        solver.run(iters)
+
+    def plain_guess(p):
+        fit._init.default_init(prob.model, prob.consts, prob.data_t, prob.lb_t, prob.ub_t,
+                               out=p)
+
+    ms_plain = _time_replays(lambda: fit_step(100, plain_guess), args)
+    f_plain = solver.final_f().double()[0]
+    med = f_plain.median()
+    ok = torch.isfinite(f_plain) & (f_plain <= 100 * med)
+    plain_ssr = float(f_plain[ok].sum())
+    its = int(solver.n_iter_t.sum())
+    conv = int((solver.step_type_t == 0).sum())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    plain_guess(pars)
+    solver.reset_state()
+    e0.record(stream)
+    solver.run(100)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1)
+    tflops = ((its - conv) * fl["F_it"] + conv * fl["F_conv"]) / (k_ms * 1e-3) / 1e12
+    log(f"DKI plain: {ms_plain:.2f} ms/step, LM kernel {k_ms:.2f} ms, {tflops:.2f} TFLOP/s, "
+        f"{int((~ok).sum())} voxels excluded")
+
+    # the initialiser: trained on forward-model signals of a separate seed
+    net = mlp.dki_mlp(w["consts"])
+    t0 = time.perf_counter()
+    T = synth.dki_truth_gpu(args.dki_train_samples, seed=20240614)
+    data, truth = net.synthesize(args.dki_train_samples, seed=777, snr=(30.0, 30.0), truth=T)
+    del T
+    net.fit_standardization(truth, data)
+    X, Tt = net.prepare(data, truth)
+    del data, truth
+    hist = net.train(X, Tt, epochs=args.dki_train_epochs, batch=8192, log=log)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del X, Tt
+
+    pd_buf = torch.empty_like(pars)
+
+    def mlp_guess(p, pr=prob, pd=None):
+        pd = pd_buf if pd is None else pd
+        fit._init.default_init(pr.model, pr.consts, pr.data_t, pr.lb_t, pr.ub_t, out=pd)
+        p.copy_(pd)
+        net.infer(pr.data_t, pr.lb_t, pr.ub_t, p)
+        _init.keep_better(pr.model, pr.consts, pr.data_t, p, pd)
+
+    # validation volume: iteration caps for both guesses
+    wv = synth.dki(1 << 18, seed=99)
+    pv = fit.make_problem("dki", wv["consts"], wv["data"])
+    ref = fit.fit(pv, iters=100, **kn)
+    fr = ref.final_f().double()[0]
+    okv = torch.isfinite(fr) & (fr <= 100 * fr.median())
+    ref_ssr = float(fr[okv].sum())
+    gm = torch.empty((22, pv.N), dtype=torch.float32, device=pars.device)
+    mlp_guess(gm, pv, torch.empty_like(gm))
+    gd = fit.default_init(pv)
+    caps = {}
+    for name, g in (("mlp", gm), ("default", gd)):
+        caps[name] = ITER_CAPS[-1]
+        for c in ITER_CAPS:
+            s = fit.fit(pv, iters=c, params0=g.clone(), **kn)
+            f = s.final_f().double()[0]
+            if bool(torch.isfinite(f[okv]).all()) and float(f[okv].sum()) <= ref_ssr * (1 + 1e-4):
+                caps[name] = c
+                break
+    log(f"DKI validated caps: {caps}")
+
+    res = {}
+    for name, guess in (("mlp_lm", mlp_guess), ("default_init_capped", plain_guess)):
+        cap = caps["mlp" if name == "mlp_lm" else "default"]
+        ms = _time_replays(lambda: fit_step(cap, guess), args)
+        f = solver.final_f().double()[0]
+        ssr = float(f[ok].sum())
+        res[name] = {"iter_cap": cap, "ms_per_step": ms, "value": N / (ms * 1e-3),
+                     "total_final_ssr": ssr, "total_ssr_rel_vs_plain": ssr / plain_ssr - 1.0,
+                     "matched_total_residual": bool(ssr <= plain_ssr * (1 + 1e-4))}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    infer_ms = e0.elapsed_time(e1)
+    mlp_flops = 2.0 * N * (61 * 256 + 256 * 256 + 256 * 22)
+    res["mlp_lm"].update({"train_samples": args.dki_train_samples,
+                          "train_epochs": args.dki_train_epochs, "train_seconds": train_s,
+                          "train_loss": hist, "inference_ms": infer_ms,
+                          "inference_tflops": mlp_flops / (infer_ms * 1e-3) / 1e12})
+    log(f"DKI: plain {ms_plain:.2f} ms, MLP-init {res['mlp_lm']['ms_per_step']:.2f} ms "
+        f"(cap {caps['mlp']}), default capped {res['default_init_capped']['ms_per_step']:.2f} ms")
+    return {"workload": "DKI 22 parameters, 2,097,152 voxels, 61 measurements, SNR 30",
+            "voxels": N, "kernel": "pca_lm_glm (wide-model kernel)", "tuning": kn,
+            "plain_lm": {"ms_per_step": ms_plain, "value": N / (ms_plain * 1e-3),
+                         "total_final_ssr": plain_ssr, "iterations_per_voxel": its / N,
+                         "lm_kernel_ms": k_ms, "lm_tflops": tflops,
+                         "lm_frac_of_fp32_peak": tflops * 1e12 / PEAK_FP32,
+                         "excluded_voxels": int((~ok).sum())},
+            **res, "speedup_vs_plain_lm": ms_plain / res["mlp_lm"]["ms_per_step"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,6 +490,10 @@ def main():
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-model batch")
+    ap.add_argument("--no-4m", action="store_true", help="skip the 4M-voxel IVIM volume")
+    ap.add_argument("--no-dki", action="store_true", help="skip the kurtosis-model volume")
+    ap.add_argument("--dki-train-samples", type=int, default=10_000_000)
+    ap.add_argument("--dki-train-epochs", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): voxel fits need no "
@@ -382,7 +586,11 @@ def main():
     if not args.plain_only:
         mlp_info = run_mlp(args, prob, knobs, plain_ssr, stream)
         value, ms_value = mlp_info.pop("value"), mlp_info.pop("ms_per_step")
+    net, cap = (mlp_info.pop("_net"), mlp_info.pop("_cap")) if mlp_info else (None, None)
+    ivim_4m = None if (args.no_4m or net is None) else run_ivim_4m(args, knobs, net, cap)
+    del net
     mixed = None if args.no_mixed else run_mixed(args, stream)
+    dki = None if args.no_dki else run_dki(args, stream)
 
     result = {
         "metric": METRIC, "value": value, "unit": "voxel-fits/s", "n_gpus": 1,
@@ -402,7 +610,9 @@ def main():
                      "iterations_per_voxel": it_sum / N},
         "mlp_lm": mlp_info,
         "speedup_vs_plain_lm": (value / plain_value) if mlp_info else None,
+        "ivim_4m": ivim_4m,
         "mixed_batch": mixed,
+        "dki": dki,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
                      "kernel": "pca_lm_chunk (all chunk launches of one fit)",

@@ -90,6 +90,47 @@ pca_eval(pca_eval_args a, pca_uconsts uc) {
     }
   }
 }
+
+#ifndef PCA_CONSTS_PER_VOXEL
+// pca_pick_init: of two initial guesses keep, per voxel, the one with the
+// lower SSR: a (in/out) is replaced by b unless SSR(a) <= SSR(b) (so a
+// non-finite SSR of a always yields b). Used to guard a learned initial guess
+// with the default one. Voxel-uniform consts.
+struct pca_pick_args {
+  real* a;            // (NP, N) in/out
+  const real* b;      // (NP, N)
+  const real* data;   // (M, N)
+  long long N;
+};
+
+extern "C" __global__ void __launch_bounds__(PCA_BLOCK)
+pca_pick_init(pca_pick_args a, pca_uconsts uc) {
+  constexpr int NP = Model::NP, NC = Model::NC;
+  const long long N = a.N;
+  const long long v = (long long)blockIdx.x * PCA_BLOCK + threadIdx.x;
+  if (v >= N) return;
+  real pa[NP], pb[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    pa[j] = a.a[(long long)j * N + v];
+    pb[j] = a.b[(long long)j * N + v];
+  }
+  real fa = 0, fb = 0;
+  for (int i = 0; i < PCA_M; ++i) {
+    real c[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) c[k] = PCA_ECONST(k, i);
+    const real y = a.data[(long long)i * N + v];
+    const real ra = Model::resid(pa, c, y), rb = Model::resid(pb, c, y);
+    fa += ra * ra;
+    fb += rb * rb;
+  }
+  if (!(fa <= fb)) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) a.a[(long long)j * N + v] = pb[j];
+  }
+}
+#endif
 #undef PCA_ECONST
 
 // ---------------------------------------------------------------------------

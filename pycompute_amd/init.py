@@ -101,3 +101,34 @@ def default_init(model: ModelSpec, consts, data_t, lb_t, ub_t, out=None, stream=
 
 
 __all__ = ["default_init", "init_matrix", "ivim_split", "IVIM", "T2", "SE", "DKI"]
+
+
+class PickArgs(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("data", ctypes.c_void_p),
+                ("N", ctypes.c_int64)]
+
+
+def keep_better(model: ModelSpec, consts, data_t, params_t, other_t, stream=None):
+    """Per voxel, replace params_t (n, N) by other_t where other_t has the
+    lower SSR on data_t (or params_t's SSR is not finite): kernel
+    pca_pick_init of the model's module (voxel-uniform consts)."""
+    from . import lm
+    consts = np.asarray(consts, np.float64)
+    if consts.ndim == 1:
+        consts = consts[None, :]
+    M = int(data_t.shape[0])
+    dt = lm.dtype_name(data_t.dtype)
+    prog = lm.get_program(lm.LMSpec(model.expr, tuple(model.pars), tuple(model.consts), M, dt))
+    for t in (data_t, params_t, other_t):
+        if not t.is_contiguous():
+            raise ValueError("keep_better inputs must be contiguous (rows, N)")
+    N = int(data_t.shape[1])
+    ucbuf, ucbytes = lm.uconsts_buffer(consts.reshape(-1), dt)
+    a = PickArgs(params_t.data_ptr(), other_t.data_ptr(), data_t.data_ptr(), N)
+    params = (ctypes.c_void_p * 2)(ctypes.cast(ctypes.pointer(a), ctypes.c_void_p),
+                                   ctypes.cast(ucbuf, ctypes.c_void_p))
+    s = torch.cuda.current_stream(data_t.device).cuda_stream if stream is None else stream
+    block = lm.DEFAULT_BLOCK
+    _lib.check(_lib.lib().pca_launch(prog.kernel("pca_pick_init"), (N + block - 1) // block, 1,
+                                     1, block, 1, 1, 0, s, params), "pca_pick_init")
+    return params_t

@@ -322,3 +322,11 @@ def ivim_mlp(consts, hidden=128, device="cuda", seed=0) -> MLPInit:
     from .models import IVIM
     return MLPInit(IVIM, consts, hidden=hidden, norm_index=0, scale_param=0,
                    log_params=(2, 3), device=device, seed=seed)
+
+
+def dki_mlp(consts, hidden=256, device="cuda", seed=0) -> MLPInit:
+    """The kurtosis-model initialiser (61 -> 256 -> 256 -> 22): signals
+    relative to the b=0 measurement, lnS0 relative to its log."""
+    from .models import DKI
+    return MLPInit(DKI, consts, hidden=hidden, norm_index=0, logshift_param=0, device=device,
+                   seed=seed)

@@ -134,3 +134,33 @@ def mixed(n_t2=400_000, n_ivim=400_000, n_se=200_000, seed=7, snr=40.0):
     each with its own seeded stream."""
     return {"t2": t2(n_t2, seed, snr), "ivim": ivim(n_ivim, B_C2, seed + 1, snr),
             "se": se(n_se, seed + 2, snr)}
+
+
+def dki_truth_gpu(N, seed, device="cuda"):
+    """The kurtosis-model parameter distribution of dki_truth, drawn on the GPU
+    with a seeded torch generator (training sets of millions of voxels); float64
+    (22, N)."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    kw = dict(generator=gen, device=device, dtype=torch.float64)
+    q = torch.randn((N, 4), **kw)
+    q = q / q.norm(dim=1, keepdim=True)
+    w, x, y, z = q.T
+    R = torch.stack([
+        torch.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        torch.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        torch.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+    ev = 0.1e-3 + (3e-3 - 0.1e-3) * torch.rand((N, 3), **kw)
+    D = torch.einsum("nij,nj,nkj->nik", R, ev, R)
+    md = ev.mean(dim=1)
+    k0 = 2.0 * torch.rand(N, **kw)
+    W = torch.zeros((N, 15), device=device, dtype=torch.float64)
+    W[:, 0:3] = k0[:, None]
+    W[:, 9:12] = k0[:, None] / 3.0
+    W += 0.1 * k0[:, None] * (2.0 * torch.rand((N, 15), **kw) - 1.0)
+    p = torch.empty((22, N), device=device, dtype=torch.float64)
+    p[0] = torch.log(0.8 + 0.4 * torch.rand(N, **kw))
+    p[1:7] = torch.stack([D[:, 0, 0], D[:, 1, 1], D[:, 2, 2], D[:, 0, 1], D[:, 0, 2], D[:, 1, 2]])
+    p[7:] = (md ** 2)[None, :] * W.T
+    return p
