@@ -290,7 +290,7 @@ def run_mixed(args, stream):
     # the same knobs, one model at a time (each its own captured graph)
     solo = []
     for p, q in zip(probs, p0):
-        s = fit.make_solver(p, compaction=True,
+        s = fit.make_solver(p, compaction=True, keep_workspace=False,
                             **{k: v for k, v in knobs.items() if k != "waves_per_eu"})
         s.setup(q.clone(), p.consts_t, p.data_t, p.lb_t, p.ub_t)
         s.capture_graph(100, init=lambda st, p=p, s=s: fit._init.default_init(
@@ -490,6 +490,11 @@ def main():
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-model batch")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="JSON knob set for the IVIM fit instead of the autotuner's choice")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a 256 MiB device copy kernel three times (a known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--no-4m", action="store_true", help="skip the 4M-voxel IVIM volume")
     ap.add_argument("--no-dki", action="store_true", help="skip the kurtosis-model volume")
     ap.add_argument("--dki-train-samples", type=int, default=10_000_000)
@@ -505,6 +510,14 @@ def main():
     from pycompute_amd import autotune, fit, synth
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)     # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 3 x elementwise copy of 256 MiB")
     w = synth.ivim_c2(args.voxels, seed=2)
     prob = fit.make_problem("ivim", w["consts"], w["data"], device=dev)
     N, M = prob.N, prob.M
@@ -520,13 +533,18 @@ def main():
         s.reset_state()
 
     def make(**kn):
-        s = fit.make_solver(prob, **kn)
+        s = fit.make_solver(prob, **dict({"keep_workspace": False}, **kn))
         setup(s)
         return s
 
-    knobs = autotune.tune(make, restore, prob.model.expr, M, "float32",
-                          allow_fast=not args.reference_rounding, log=log)
+    if args.tuning:
+        knobs = dict(autotune.DEFAULT, **json.loads(args.tuning))
+    else:
+        knobs = autotune.tune(make, restore, prob.model.expr, M, "float32",
+                              allow_fast=not args.reference_rounding, log=log)
     log(f"tuned: {knobs}")
+    # the fits keep no per-voxel workspace copies (g, h, hl, step): results only
+    knobs = dict(knobs, keep_workspace=False)
     solver = make(compaction=True, **knobs)
     flops = solver.program.flops
     pars = solver.pars_t

@@ -10,6 +10,7 @@ of the caller's own problem:
                    per-voxel data arrays in scratch, so full wins for small n)
   * fast_exp / fast_div: v_exp_f32 and the approximate FP32 divide (only when
                    the caller allows non-reference rounding)
+  * reassoc:       FMA-folded Hessian update and reciprocal-based solve (idem)
   * waves_per_eu:  occupancy hint (mixed-model batches, batch_candidates())
 
 The fastest candidate whose results agree with the reference-rounding build on
@@ -27,7 +28,8 @@ import torch
 
 from . import program
 
-DEFAULT = {"block": 256, "chunk": 16, "unroll": 0, "fast_exp": False, "fast_div": False}
+DEFAULT = {"block": 256, "chunk": 16, "unroll": 0, "fast_exp": False, "fast_div": False,
+           "reassoc": False, "waves_per_eu": 0}
 
 
 def _cache_path() -> str:
@@ -56,13 +58,19 @@ def key(expr: str, M: int, dtype: str, allow_fast: bool, arch: str) -> str:
 
 
 def candidates(nparam: int, allow_fast: bool):
+    """The single-model knob sets. The FMA-folded build runs with an occupancy
+    hint of 2 waves per SIMD too: it needs a few registers more than 256,
+    which would otherwise halve occupancy."""
     out = []
-    fast = [(False, False)] + ([(True, True), (True, False)] if allow_fast else [])
-    for fe, fd in fast:
+    fast = [(False, False, False, 0)]
+    if allow_fast:
+        fast += [(True, True, False, 0), (True, False, False, 0), (True, True, True, 0),
+                 (True, True, True, 2)]
+    for fe, fd, ra, wpe in fast:
         for chunk in (8, 16, 24):
             for block in (256, 128):
                 out.append({"block": block, "chunk": chunk, "unroll": 0, "fast_exp": fe,
-                            "fast_div": fd})
+                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe})
     return out
 
 
@@ -71,12 +79,13 @@ def batch_candidates(allow_fast: bool = True):
     register allocation is the largest model's; the occupancy hint (waves per
     SIMD) trades that against spills."""
     out = []
-    fast = [(False, False)] + ([(True, True)] if allow_fast else [])
-    for fe, fd in fast:
+    fast = [(False, False, False)] + ([(True, True, False), (True, True, True)]
+                                      if allow_fast else [])
+    for fe, fd, ra in fast:
         for chunk in (8, 16):
             for wpe in (0, 2):
                 out.append({"block": 256, "chunk": chunk, "unroll": 0, "fast_exp": fe,
-                            "fast_div": fd, "waves_per_eu": wpe})
+                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe})
     return out
 
 

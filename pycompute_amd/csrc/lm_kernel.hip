@@ -79,6 +79,10 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
     for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
     Mdl::accum(V.p, c, V.y[i], V.lam, V.f, V.g, V.h, V.hld);
   }
+  if constexpr (Mdl::HLD_DAMP_ONLY) {   // the fast build accumulates damping only
+#pragma unroll
+    for (int j = 0; j < NP; ++j) V.hld[j] += V.h[tri(j, j)];
+  }
 
   // step: hl s = g, then scrub NaN/Inf (lsqnonlin.py:843-891)
   gmw81_solve<NP>(V.h, V.hld, V.g, V.s);

@@ -53,6 +53,14 @@ __host__ __device__ constexpr int perm(int n, int i) { return i == 0 ? n - 1 : i
 template <int n>
 __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, const real* g,
                                             real* s) {
+  // PCA_FAST_SOLVE (fast builds): every division becomes a multiplication by
+  // a hardware reciprocal computed once per divisor (1/sc_i = rsqrt|B_ii|,
+  // 1/d_j, 1/beta^2); results equal up to rounding.
+#ifdef PCA_FAST_SOLVE
+  constexpr bool FAST = true;
+#else
+  constexpr bool FAST = false;
+#endif
   constexpr int nh = n * (n + 1) / 2;
   real B[nh], b[n], sc[n];
 #pragma unroll
@@ -66,13 +74,23 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
   }
 #pragma unroll
   for (int i = 0; i < n; ++i) {
-    sc[i] = pca_sqrt(pca_abs(B[tri(i, i)]));
-    b[i] /= sc[i];
+    if constexpr (FAST) {
+      sc[i] = pca_rcp_fast(pca_sqrt_fast(pca_abs(B[tri(i, i)])));   // 1 / scale
+      b[i] *= sc[i];
+    } else {
+      sc[i] = pca_sqrt(pca_abs(B[tri(i, i)]));
+      b[i] /= sc[i];
+    }
   }
 #pragma unroll
   for (int i = 0; i < n; ++i)
 #pragma unroll
-    for (int j = 0; j <= i; ++j) B[tri(i, j)] /= (sc[i] * sc[j]);
+    for (int j = 0; j <= i; ++j) {
+      if constexpr (FAST)
+        B[tri(i, j)] *= (sc[i] * sc[j]);
+      else
+        B[tri(i, j)] /= (sc[i] * sc[j]);
+    }
 
   // GMW81
   real gamma = 0, xi = 0;
@@ -87,16 +105,28 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
         if (t > xi) xi = t;
       }
     }
-  if (n > 1) xi /= pca_sqrt((real)(n * n - 1));
+  if (n > 1) {
+    if constexpr (FAST)
+      xi *= (real)1 / pca_sqrt((real)(n * n - 1));
+    else
+      xi /= pca_sqrt((real)(n * n - 1));
+  }
   real beta2 = PCA_EPS_GMW, delta = PCA_EPS_GMW;
   if (beta2 < gamma) beta2 = gamma;
   if (beta2 < xi) beta2 = xi;
   if (gamma + xi > (real)1) delta *= (gamma + xi);
+  real rbeta2 = 0, rd[n];
+  if constexpr (FAST) rbeta2 = pca_rcp_fast(beta2);
 
 #pragma unroll
   for (int j = 0; j < n; ++j) {
 #pragma unroll
-    for (int q = 0; q < j; ++q) B[tri(j, q)] /= B[tri(q, q)];
+    for (int q = 0; q < j; ++q) {
+      if constexpr (FAST)
+        B[tri(j, q)] *= rd[q];
+      else
+        B[tri(j, q)] /= B[tri(q, q)];
+    }
 #pragma unroll
     for (int i = j + 1; i < n; ++i) {
       real t = B[tri(i, j)];
@@ -113,13 +143,17 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
     theta *= theta;
     real d = pca_abs(B[tri(j, j)]);
     if (d < delta) d = delta;
-    const real t0 = theta / beta2;
+    const real t0 = FAST ? theta * rbeta2 : theta / beta2;
     if (d < t0) d = t0;
     B[tri(j, j)] = d;
+    if constexpr (FAST) rd[j] = pca_rcp_fast(d);
 #pragma unroll
     for (int i = j + 1; i < n; ++i) {
       const real t = B[tri(i, j)];
-      B[tri(i, i)] -= t * t / d;
+      if constexpr (FAST)
+        B[tri(i, i)] -= t * t * rd[j];
+      else
+        B[tri(i, i)] -= t * t / d;
     }
   }
 
@@ -130,7 +164,10 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
     real t = b[i];
 #pragma unroll
     for (int j = 0; j < i; ++j) t -= B[tri(i, j)] * B[tri(j, j)] * z[j];
-    z[i] = t / B[tri(i, i)];
+    if constexpr (FAST)
+      z[i] = t * rd[i];
+    else
+      z[i] = t / B[tri(i, i)];
   }
 #pragma unroll
   for (int i = n - 1; i >= 0; --i) {
@@ -140,7 +177,12 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
     z[i] = t;
   }
 #pragma unroll
-  for (int i = 0; i < n; ++i) s[perm(n, i)] = z[i] / sc[i];
+  for (int i = 0; i < n; ++i) {
+    if constexpr (FAST)
+      s[perm(n, i)] = z[i] * sc[i];
+    else
+      s[perm(n, i)] = z[i] / sc[i];
+  }
 }
 
 struct Settings {

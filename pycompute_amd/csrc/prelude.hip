@@ -52,5 +52,13 @@ __device__ __forceinline__ float pca_abs(float a) { return fabsf(a); }
 __device__ __forceinline__ double pca_abs(double a) { return fabs(a); }
 __device__ __forceinline__ float pca_sqrt(float a) { return sqrtf(a); }
 __device__ __forceinline__ double pca_sqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float pca_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double pca_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32: about 1 ulp,
+// no denormal scaling), used by the fast solve (PCA_FAST_SOLVE).
+__device__ __forceinline__ float pca_rcp_fast(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double pca_rcp_fast(double a) { return 1.0 / a; }
+__device__ __forceinline__ float pca_sqrt_fast(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double pca_sqrt_fast(double a) { return sqrt(a); }
 __device__ __forceinline__ bool pca_nonfinite(float a) { return __builtin_isnan(a) || __builtin_isinf(a); }
 __device__ __forceinline__ bool pca_nonfinite(double a) { return __builtin_isnan(a) || __builtin_isinf(a); }

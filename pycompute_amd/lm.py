@@ -51,6 +51,9 @@ class LMSpec:
     unroll: int = 0
     waves_per_eu: int = 0
     fast_div: bool = False
+    # reassociated FMA form of the Hessian update and a reciprocal-based
+    # solve (fast builds; results equal the reference's up to rounding)
+    reassoc: bool = False
 
     @property
     def nparam(self):
@@ -93,8 +96,8 @@ class LMProgram:
             self.funcid = f"batched_levenberg_marquardt_{len(ids.split('|'))}_" \
                           f"{hipgen.expr_hash(ids, 12)}"
         self.source = build_source(spec, self.dag, list(zip(self.members, self.member_dags)))
-        if spec.strict and (spec.fast_exp or spec.fast_div):
-            raise ValueError("the strict build excludes fast_exp and fast_div")
+        if spec.strict and (spec.fast_exp or spec.fast_div or spec.reassoc):
+            raise ValueError("the strict build excludes fast_exp, fast_div and reassoc")
         options = list(BASE_OPTIONS)
         if spec.strict:
             options.append("-ffp-contract=off")
@@ -148,11 +151,13 @@ def build_source(spec: LMSpec, dag, members=()) -> str:
         head.append("#define PCA_STRICT 1")
     if spec.waves_per_eu > 0:
         head.append(f"#define PCA_WAVES_PER_EU {spec.waves_per_eu}")
+    if spec.reassoc:
+        head.append("#define PCA_FAST_SOLVE 1")
     structs, typedefs = {}, []
     for i, (s, d) in enumerate(models):
         sname = "PcaModel_" + hipgen.expr_hash(s.expr, 12)
         if sname not in structs:
-            structs[sname] = hipgen.emit_model(d, sname, spec.dtype)
+            structs[sname] = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc)
         if members:
             typedefs.append(f"typedef {sname} Model{i};")
         if i == 0:

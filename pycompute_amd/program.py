@@ -104,10 +104,28 @@ def cache_key(source: str, options) -> str:
     return h.hexdigest()[:32]
 
 
+_USED_CACHE_FILES: set = set()
+
+
+def prune_cache() -> int:
+    """Delete cached code objects this process neither compiled nor loaded
+    (stale sources); returns the number removed. Called at the end of a build
+    that has compiled every shipped workload."""
+    d = cache_dir()
+    removed = 0
+    for fn in os.listdir(d):
+        path = os.path.join(d, fn)
+        if fn.endswith(".hsaco") and path not in _USED_CACHE_FILES:
+            os.remove(path)
+            removed += 1
+    return removed
+
+
 def compile_cached(source: str, name: str, options=None) -> bytes:
     """hiprtc-compile ``source`` (gfx950), reusing a cached code object."""
     options = list(BASE_OPTIONS if options is None else options)
     path = os.path.join(cache_dir(), f"{name}-{cache_key(source, options)}.hsaco")
+    _USED_CACHE_FILES.add(path)
     if os.path.exists(path):
         with open(path, "rb") as fh:
             return fh.read()

@@ -176,14 +176,23 @@ class _Printer:
         return result
 
 
-def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32") -> str:
-    """HIP source of the model struct (see module docstring)."""
+def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
+               reassoc: bool = False) -> str:
+    """HIP source of the model struct (see module docstring).
+
+    reassoc=False keeps the reference's association of the per-measurement
+    update: hj = d2f*r + J_j J_k; h += hj; hld += hj + max(lam J_j^2, 1e-4).
+    reassoc=True (fast builds) folds it into FMAs, h = fma(J_j, J_k,
+    fma(d2f, r, h)), and accumulates only the damping terms into hld (the
+    kernel adds the diagonal of h after the measurement loop,
+    HLD_DAMP_ONLY): fewer instructions, results equal up to rounding."""
     n, nh = dag.nparam, dag.nhes
     floor = _lit(1e-4, dtype)
     out = [f"struct {struct_name} {{",
            f"  static constexpr int NP = {n};",
            f"  static constexpr int NC = {dag.nconst};",
-           f"  static constexpr int NH = {nh};"]
+           f"  static constexpr int NH = {nh};",
+           f"  static constexpr bool HLD_DAMP_ONLY = {'true' if reassoc else 'false'};"]
 
     # resid: model - y (the F kernel body, lsqnonlin.py:60-66)
     pr = _Printer(dag, dtype)
@@ -206,15 +215,25 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32") -> str:
         Jj = pr.ref(dag.jac[j])
         for k in range(j + 1):
             Jk = pr.ref(dag.jac[k])
-            if nz[l]:
-                hj = f"{pr.ref(dag.hes[l])}*r + jt{l}"
+            if reassoc:
+                acc = f"pca_fma({pr.ref(dag.hes[l])}, r, h[{l}])" if nz[l] else f"h[{l}]"
+                if j == k:
+                    body.append(f"{{ const real jt{l} = {Jj}*{Jj}; h[{l}] = {acc} + jt{l}; "
+                                f"hld[{j}] += pca_fmax(lam*jt{l}, {floor}); }}")
+                else:
+                    body.append(f"h[{l}] = pca_fma({Jj}, {Jk}, {acc});")
             else:
-                hj = f"jt{l}"
-            body.append(f"{{ const real jt{l} = {Jj}*{Jk}; const real hj = {hj}; h[{l}] += hj;"
-                        + (f" hld[{j}] += hj + pca_fmax(lam*jt{l}, {floor});" if j == k else "")
-                        + " }")
+                if nz[l]:
+                    hj = f"{pr.ref(dag.hes[l])}*r + jt{l}"
+                else:
+                    hj = f"jt{l}"
+                body.append(f"{{ const real jt{l} = {Jj}*{Jk}; const real hj = {hj}; "
+                            f"h[{l}] += hj;"
+                            + (f" hld[{j}] += hj + pca_fmax(lam*jt{l}, {floor});"
+                               if j == k else "")
+                            + " }")
             l += 1
-        body.append(f"g[{j}] += {Jj}*r;")
+        body.append(f"g[{j}] = pca_fma({Jj}, r, g[{j}]);" if reassoc else f"g[{j}] += {Jj}*r;")
     out.append("  __device__ static __forceinline__ void accum(const real* __restrict__ p, "
                "const real* __restrict__ c, real y, real lam, real& f, real* g, real* h, "
                "real* hld) {")

@@ -37,6 +37,8 @@ class GLMProgram:
             head.append("#define PCA_FAST_EXP 1")
         if self.form.phi == "identity":
             head.append("#define PCA_GLM_IDENTITY 1")
+        if spec.reassoc:
+            head.append("#define PCA_FAST_SOLVE 1")
         self.source = "\n".join(["\n".join(head), read_csrc("strict_math.h"),
                                  read_csrc("prelude.hip"), read_csrc("lm_solver.hip"),
                                  read_csrc("lm_glm.hip")])

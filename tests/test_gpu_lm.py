@@ -38,10 +38,10 @@ def _summary(name, x):
     print(f"{name}: median {q[0]:.2e} p99 {q[1]:.2e} max {q[2]:.2e}", flush=True)
 
 
-def _run_pair(model, w, iters, dtype=torch.float32, n=None):
+def _run_pair(model, w, iters, dtype=torch.float32, n=None, **kw):
     prob = fit.make_problem(model, w["consts"], w["data"][:, :n] if n else w["data"], dtype=dtype)
     p0 = fit.default_init(prob)
-    solver = fit.fit(prob, iters=iters, params0=p0.clone())
+    solver = fit.fit(prob, iters=iters, params0=p0.clone(), **kw)
     torch.cuda.synchronize()
     npd = np.float32 if dtype == torch.float32 else np.float64
     ref = oracle.lm_run(prob.model.name, prob.consts, prob.data_t.cpu().numpy(),
@@ -91,6 +91,23 @@ def test_t2_and_se_parity(gpu):
     for model, w in (("t2", synth.t2(8192, seed=3)), ("se", synth.se(8192, seed=4))):
         prob, p0, g, r = _run_pair(model, w, 100)
         _check(g, r, model)
+
+
+FAST_BUILDS = {"fast": dict(fast_exp=True, fast_div=True),
+               "fast-reassoc": dict(fast_exp=True, fast_div=True, reassoc=True)}
+
+
+@pytest.mark.parametrize("build", sorted(FAST_BUILDS))
+def test_fast_builds_parity(gpu, build):
+    """The autotuner's faster builds (v_exp_f32, approximate divide, FMA-folded
+    Hessian update and reciprocal solve) meet the same bars."""
+    kw = FAST_BUILDS[build]
+    w = synth.ivim(65536, synth.B_C2, seed=2, snr=(20.0, 100.0))
+    prob, p0, g, r = _run_pair("ivim", w, 100, **kw)
+    _check(g, r, f"ivim-64k-{build}")
+    for model, wm in (("t2", synth.t2(8192, seed=3)), ("se", synth.se(8192, seed=4))):
+        prob, p0, g, r = _run_pair(model, wm, 100, **kw)
+        _check(g, r, f"{model}-{build}")
 
 
 def test_ivim_float64_parity(gpu):
