@@ -80,6 +80,20 @@ IVIM_BOX = ([0.8, 0.05, 5e-3, 0.5e-3], [1.2, 0.4, 0.1, 3e-3])   # training param
 ITER_CAPS = (2, 3, 4, 5, 6, 8, 10, 12, 15, 20, 25, 30, 40, 50, 70, 100)
 
 
+def _traffic(knobs):
+    """HBM bytes per fit of the chunked LM kernel, from the copy-calibrated
+    FETCH_SIZE / WRITE_SIZE counter runs recorded in profiles/ for the same
+    knob set (None when the knobs differ or no record exists)."""
+    path = os.path.join(ROOT, "profiles", "pca_lm_chunk_ivim_1M.traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    k = {key: v for key, v in knobs.items() if key != "keep_workspace"}
+    return rec["bytes_per_fit"] if rec.get("tuning") == k else None
+
+
 def _time_replays(replay, args):
     """ms per call of replay() over args.steps calls after args.warmup."""
     import torch
@@ -632,7 +646,9 @@ def main():
         "mixed_batch": mixed,
         "dki": dki,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": _traffic(knobs), "traffic_unit": "bytes per fit (all chunks)",
+                     "algorithmic_bytes": 137 * N,
                      "kernel": "pca_lm_chunk (all chunk launches of one fit)",
                      "kernel_ms": k_avg, "flops_per_iteration": flops["F_it"],
                      "exp_per_iteration": flops["exp_it"],
