@@ -201,6 +201,12 @@ int pca_launch(pca_kernel kernel, uint32_t gx, uint32_t gy, uint32_t gz, uint32_
 int pca_bpolyfit(const void* x, int32_t x_per_voxel, const void* y, int32_t m, int32_t deg,
                  void* out, int64_t N, int32_t is_double, void* stream);
 
+/* Canonical IVIM labelling of fitted parameters (4, N) = (S0, f, D_1, D_2),
+ * in place: where D_1 < D_2, swap D_1 and D_2 and set f = 1 - f. Replaces the
+ * post-fit cell of test/ivim/test_ivim.ipynb:441-446. is_double selects the
+ * element type. */
+int pca_ivim_canonicalize(void* params, int64_t N, int32_t is_double, void* stream);
+
 /* Default initial guess: out (n, N) = clip(nan_to_num(T(P log y)), lb, ub)
  * with P a device (K, M) double matrix shared by all voxels and T chosen by
  * kind (0 identity, 1 mono-exponential, 2 stretched exponential, 3 segmented

@@ -223,3 +223,159 @@ pca_synth(pca_synth_args a, pca_uconsts uc) {
     a.data[(long long)i * a.N + v] = (real)sqrt(re * re + im * im);
   }
 }
+
+// ---------------------------------------------------------------------------
+// Random search (pycompute/cuda/lsqnonlin.py:1082-1242: SearchStep,
+// RandomSearch, SecondOrderRandomSearch). Candidates are uniform in
+// [lb, ub] per voxel; candidate k of voxel v takes its NP uniforms from
+// Philox4x32-10 with key = seed and counter (v, v >> 32, k, 0x7a110000 + j/4)
+// (u = (x + 0.5) 2^-32), so a search can be continued and replayed. A
+// candidate's score is its own SSR (the reference accumulates every
+// candidate's SSR into one never-cleared f, defect D11).
+// ---------------------------------------------------------------------------
+
+#ifdef PCA_CONSTS_PER_VOXEL
+#define PCA_RCONST(k, i) a.consts[((long long)(k) * PCA_M + (i)) * N + v]
+#else
+#define PCA_RCONST(k, i) uc.c[(k) * PCA_M + (i)]
+#endif
+
+namespace pca {
+
+// the NP uniforms of candidate k of voxel v, scaled to [lo, hi] as
+// lo + (hi - lo) u with separate roundings (no FMA contraction, whatever the
+// build allows), so the host can replay the candidates bit for bit
+template <int NP>
+__device__ __forceinline__ void random_candidate(unsigned long long seed, long long v,
+                                                 long long k, const real* lo, const real* hi,
+                                                 real* p) {
+#pragma clang fp contract(off)
+  const unsigned k0 = (unsigned)seed, k1 = (unsigned)(seed >> 32);
+#pragma unroll
+  for (int j0 = 0; j0 < NP; j0 += 4) {
+    const U4 c{(unsigned)v, (unsigned)((unsigned long long)v >> 32), (unsigned)k,
+               0x7a110000u + (unsigned)(j0 / 4)};
+    const U4 r = philox4x32_10(c, k0, k1);
+    const unsigned w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (j0 + q < NP) p[j0 + q] = lo[j0 + q] + (hi[j0 + q] - lo[j0 + q]) * (real)u01(w[q]);
+  }
+}
+
+template <class CF>
+__device__ __forceinline__ real ssr_at(const real* p, const real* data, long long N, long long v,
+                                       const CF& cf) {
+  constexpr int NC = Model::NC;
+  real f = 0;
+  for (int i = 0; i < PCA_M; ++i) {
+    real c[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+    const real r = Model::resid(p, c, data[(long long)i * N + v]);
+    f += r * r;
+  }
+  return f;
+}
+
+}  // namespace pca
+
+struct pca_rsearch_args {
+  const real* data;     // (M, N)
+  const real* consts;   // (NC*M, N) when per-voxel
+  const real* lb;       // (NP, N)
+  const real* ub;       // (NP, N)
+  real* best_p;         // (NP, N) in/out
+  real* best_f;         // (N) in/out
+  unsigned long long seed;
+  long long N;
+  long long draw0;      // index of the first candidate of this call
+  int ncand;            // candidates per voxel in this call
+  int reserved;
+};
+
+// RandomSearch.run: ncand candidates per voxel in one launch, best kept.
+extern "C" __global__ void __launch_bounds__(PCA_BLOCK)
+pca_random_search(pca_rsearch_args a, pca_uconsts uc) {
+  constexpr int NP = Model::NP;
+  const long long N = a.N;
+  const long long v = (long long)blockIdx.x * PCA_BLOCK + threadIdx.x;
+  if (v >= N) return;
+  real lo[NP], hi[NP], bp[NP], p[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    lo[j] = a.lb[(long long)j * N + v];
+    hi[j] = a.ub[(long long)j * N + v];
+    bp[j] = a.best_p[(long long)j * N + v];
+  }
+  real bf = a.best_f[v];
+  auto cf = [&](int k, int i) { return PCA_RCONST(k, i); };
+  for (int k = 0; k < a.ncand; ++k) {
+    pca::random_candidate<NP>(a.seed, v, a.draw0 + k, lo, hi, p);
+    const real f = pca::ssr_at(p, a.data, N, v, cf);
+    if (f < bf) {
+      bf = f;
+#pragma unroll
+      for (int j = 0; j < NP; ++j) bp[j] = p[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) a.best_p[(long long)j * N + v] = bp[j];
+  a.best_f[v] = bf;
+}
+
+struct pca_rparams_args {
+  const real* lb;       // (NP, N)
+  const real* ub;       // (NP, N)
+  real* p;              // (NP, N) out: candidate `draw` of every voxel
+  unsigned long long seed;
+  long long N;
+  long long draw;
+};
+
+// The random starting points of SecondOrderRandomSearch (the same stream as
+// pca_random_search).
+extern "C" __global__ void __launch_bounds__(PCA_BLOCK) pca_random_params(pca_rparams_args a) {
+  constexpr int NP = Model::NP;
+  const long long N = a.N;
+  const long long v = (long long)blockIdx.x * PCA_BLOCK + threadIdx.x;
+  if (v >= N) return;
+  real lo[NP], hi[NP], p[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    lo[j] = a.lb[(long long)j * N + v];
+    hi[j] = a.ub[(long long)j * N + v];
+  }
+  pca::random_candidate<NP>(a.seed, v, a.draw, lo, hi, p);
+#pragma unroll
+  for (int j = 0; j < NP; ++j) a.p[(long long)j * N + v] = p[j];
+}
+
+struct pca_keep_args {
+  const real* cand;     // (NP, N) candidate parameters (an LM result)
+  const real* data;     // (M, N)
+  const real* consts;   // (NC*M, N) when per-voxel
+  real* best_p;         // (NP, N) in/out
+  real* best_f;         // (N) in/out
+  long long N;
+};
+
+// SearchStep after an LM run: keep the candidate when its SSR is lower.
+extern "C" __global__ void __launch_bounds__(PCA_BLOCK)
+pca_keep_best(pca_keep_args a, pca_uconsts uc) {
+  constexpr int NP = Model::NP;
+  const long long N = a.N;
+  const long long v = (long long)blockIdx.x * PCA_BLOCK + threadIdx.x;
+  if (v >= N) return;
+  real p[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) p[j] = a.cand[(long long)j * N + v];
+  auto cf = [&](int k, int i) { return PCA_RCONST(k, i); };
+  const real f = pca::ssr_at(p, a.data, N, v, cf);
+  if (f < a.best_f[v]) {
+    a.best_f[v] = f;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) a.best_p[(long long)j * N + v] = p[j];
+  }
+}
+#undef PCA_RCONST

@@ -152,7 +152,37 @@ k_loglinear_init(const double* __restrict__ P, int Kr, int M, const T* __restric
   }
 }
 
+// Canonical IVIM labelling (test/ivim/test_ivim.ipynb:441-446): the model is
+// symmetric under (f, D_1, D_2) -> (1 - f, D_2, D_1); where D_1 < D_2 the two
+// diffusivities are swapped and f becomes 1 - f, so D_1 (pseudo-diffusion) >=
+// D_2 everywhere. params (4, N): S0, f, D_1, D_2.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ivim_canonicalize(T* p, long long N) {
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= N) return;
+  const T d1 = p[2 * N + v], d2 = p[3 * N + v];
+  if (d1 < d2) {
+    p[2 * N + v] = d2;
+    p[3 * N + v] = d1;
+    p[N + v] = (T)1 - p[N + v];
+  }
+}
+
 }  // namespace
+
+extern "C" int pca_ivim_canonicalize(void* params, int64_t N, int32_t is_double, void* stream) {
+  if (!params || N < 0) return pca_fail(PCA_ERR_ARG, "pca_ivim_canonicalize: bad argument");
+  if (N == 0) return PCA_OK;
+  const dim3 grid((unsigned)((N + 255) / 256)), block(256);
+  if (is_double)
+    hipLaunchKernelGGL(k_ivim_canonicalize<double>, grid, block, 0, (hipStream_t)stream,
+                       (double*)params, (long long)N);
+  else
+    hipLaunchKernelGGL(k_ivim_canonicalize<float>, grid, block, 0, (hipStream_t)stream,
+                       (float*)params, (long long)N);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PCA_OK : pca_fail(PCA_ERR_HIP, hipGetErrorString(e));
+}
 
 extern "C" int pca_bpolyfit(const void* x, int32_t x_per_voxel, const void* y, int32_t m,
                             int32_t deg, void* out, int64_t N, int32_t is_double, void* stream) {

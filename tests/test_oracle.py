@@ -256,3 +256,18 @@ def test_bpolyfit_restatement_is_least_squares():
     c = rng.standard_normal((3, 32))
     y = c[0] + c[1] * x + c[2] * x ** 2
     np.testing.assert_allclose(oracle.bpolyfit(x, y, 2), c, rtol=1e-9, atol=1e-9)
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors of the Random123 distribution
+    (counter, key) -> output."""
+    from oracle.philox import philox4x32_10
+    cases = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+             ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6,
+                                                     0x6d5451fd)),
+             ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+              (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for c, k, want in cases:
+        got = philox4x32_10([np.array([x], np.uint32) for x in c],
+                            [np.array([x], np.uint32) for x in k])
+        assert tuple(int(x[0]) for x in got) == want
