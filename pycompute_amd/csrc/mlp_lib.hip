@@ -41,14 +41,20 @@ __device__ __forceinline__ floatx4 mfma16(bf16x4 a, bf16x4 b, floatx4 c) {
 // GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
 // evaluated as x * sigmoid(2u) = x / (1 + exp(-2u)) with the hardware exp and
 // reciprocal (a few ulp; the result is rounded to bf16 next anyway).
-__device__ __forceinline__ float gelu(float x) {
-  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+// GELU, tanh form: 0.5 x (1 + tanh u) = x sigmoid(2u), u = c (x + k x^3),
+// c = sqrt(2/pi), k = 0.044715. With the constants folded into one exp2
+// argument, x sigmoid(2u) = x / (1 + 2^(x (A + B x^2))), A = -2 c log2(e),
+// B = A k: five VALU operations plus v_exp_f32 and v_rcp_f32 per activation.
+constexpr float kGeluA = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+constexpr float kGeluB = kGeluA * 0.044715f;
+__device__ __forceinline__ float gelu_sigmoid(float x) {   // sigmoid(2u)
+  const float t = x * __builtin_fmaf(kGeluB, x * x, kGeluA);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
-// d/dx x s(2u) = s + 2 x s (1 - s) u',  s = sigmoid(2u)
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sigmoid(x); }
+// d/dx x s(2u) = s + 2 x s (1 - s) u',  s = sigmoid(2u), u' = c (1 + 3 k x^2)
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  const float s = gelu_sigmoid(x);
   const float du = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
   return s + 2.0f * x * s * (1.0f - s) * du;
 }
