@@ -38,9 +38,6 @@ __device__ __forceinline__ floatx4 mfma16(bf16x4 a, bf16x4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
-// evaluated as x * sigmoid(2u) = x / (1 + exp(-2u)) with the hardware exp and
-// reciprocal (a few ulp; the result is rounded to bf16 next anyway).
 // GELU, tanh form: 0.5 x (1 + tanh u) = x sigmoid(2u), u = c (x + k x^3),
 // c = sqrt(2/pi), k = 0.044715. With the constants folded into one exp2
 // argument, x sigmoid(2u) = x / (1 + 2^(x (A + B x^2))), A = -2 c log2(e),
