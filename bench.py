@@ -371,16 +371,34 @@ def run_dki(args, stream):
     default guess at its own validated cap is reported beside it. Residual
     totals leave out the voxels plain LM itself fails on (final SSR non-finite
     or above 100x the median)."""
+    import numpy as np
     import torch
-    from pycompute_amd import fit, mlp, synth
+    from pycompute_amd import autotune, fit, mlp, synth
     from pycompute_amd import init as _init
 
     rr = args.reference_rounding
-    kn = {"fast_exp": not rr, "fast_div": not rr}
     w = synth.dki(1 << 21, seed=6)
     prob = fit.make_problem("dki", w["consts"], w["data"])
     N = prob.N
     p0 = fit.default_init(prob)
+    # wide-kernel knobs (tile = register/LDS split) tuned on a 262,144-voxel sample
+    ns = 1 << 18
+    sub = fit.make_problem("dki", w["consts"], np.ascontiguousarray(w["data"][:, :ns]))
+    p0s = p0[:, :ns].contiguous()
+
+    def make_sub(**kn):
+        s = fit.make_solver(sub, **{k: v for k, v in kn.items() if k != "compaction"})
+        s.setup(p0s.clone(), sub.consts_t, sub.data_t, sub.lb_t, sub.ub_t)
+        return s
+
+    def restore_sub(s):
+        s.pars_t.copy_(p0s)
+        s.reset_state()
+
+    kn = autotune.tune(make_sub, restore_sub, prob.model.expr, prob.M, "float32",
+                       allow_fast=not rr, log=log, iters=30, reps=2,
+                       cands=autotune.wide_candidates(not rr))
+    log(f"DKI wide kernel tuned: {kn}")
     solver = fit.make_solver(prob, **kn)
     pars = torch.empty((22, N), dtype=torch.float32, device=prob.data_t.device)
     solver.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)

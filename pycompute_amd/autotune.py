@@ -12,6 +12,7 @@ of the caller's own problem:
                    the caller allows non-reference rounding)
   * reassoc:       FMA-folded Hessian update and reciprocal-based solve (idem)
   * waves_per_eu:  occupancy hint (mixed-model batches, batch_candidates())
+  * tile:          wide-model kernel register/LDS split (wide_candidates())
 
 The fastest candidate whose results agree with the reference-rounding build on
 the sample (final SSR within 1e-4 relative for at least as many voxels as the
@@ -86,6 +87,19 @@ def batch_candidates(allow_fast: bool = True):
             for wpe in (0, 2):
                 out.append({"block": 256, "chunk": chunk, "unroll": 0, "fast_exp": fe,
                             "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe})
+    return out
+
+
+def wide_candidates(allow_fast: bool = True):
+    """Knobs of the wide-model kernel: the tile size sets the register/LDS
+    layout (entries per accumulation pass kept in VGPRs; the other tiles are
+    parked in LDS, which bounds occupancy)."""
+    out = []
+    fast = [(False, False, False)] + ([(True, True, False), (True, True, True)]
+                                      if allow_fast else [])
+    for fe, fd, ra in fast:
+        for tile in (100, 136, 170):
+            out.append({"tile": tile, "fast_exp": fe, "fast_div": fd, "reassoc": ra})
     return out
 
 
