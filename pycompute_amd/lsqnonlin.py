@@ -205,8 +205,9 @@ class SecondOrderLevenbergMarquardt(HipFunction):
                 from . import wide as _wide
                 self._glm = _wide.glm_program(self._spec_uniform, self.tile)
                 if wide and self._glm is None:
-                    raise ValueError("the wide kernel needs n > 8, a non-strict build and a "
-                                     "model phi(eta) with eta affine in the parameters")
+                    raise ValueError("the wide kernel needs n > 8, a non-strict build, a "
+                                     "model phi(eta) with eta affine in the parameters and a "
+                                     "tile whose parked Hessian entries fit in 64 KB of LDS")
         else:
             self.funcid = self._plugin.funcid
         self.Nprobs = 1

@@ -14,6 +14,34 @@ from .sym import glm, hipgen
 
 WIDE_MIN_PARAMS = 9
 DEFAULT_TILE = 136
+LDS_LIMIT = 65536   # bytes of LDS one workgroup may allocate
+
+
+def tile_rows(n: int, tile_max: int) -> list:
+    """First row of each tile (and n at the end): runs of whole rows of the
+    packed lower triangle with at most tile_max entries, at least one row
+    (the host restatement of tile_row in csrc/lm_glm.hip)."""
+    rows, r = [0], 0
+    while r < n:
+        e = 0
+        while True:
+            e += r + 1
+            r += 1
+            if not (r < n and e + r + 1 <= tile_max):
+                break
+        rows.append(r)
+    return rows
+
+
+def park_entries(n: int, tile_max: int) -> int:
+    """Hessian entries parked in LDS (all tiles but the last)."""
+    r = tile_rows(n, tile_max)[-2]
+    return r * (r + 1) // 2
+
+
+def lds_bytes(n: int, tile_max: int, dtype: str) -> int:
+    """LDS of one 64-lane workgroup: one column of parked entries per lane."""
+    return max(park_entries(n, tile_max), 1) * 64 * (8 if dtype == "float64" else 4)
 
 
 class GLMProgram:
@@ -26,6 +54,10 @@ class GLMProgram:
                              "parameters")
         self.tile_max = int(tile_max)
         n = spec.nparam
+        self.lds_bytes = lds_bytes(n, self.tile_max, spec.dtype)
+        if self.lds_bytes > LDS_LIMIT:
+            raise ValueError(f"tile {self.tile_max} parks {self.lds_bytes} bytes of LDS per "
+                             f"workgroup (limit {LDS_LIMIT}); use a larger tile")
         self.stride = glm.table_stride(n)
         self.funcid = spec.funcid() + f"_glm{self.tile_max}"
         head = [f"// generated: {self.funcid}", f"// model: {spec.expr}",
@@ -87,8 +119,10 @@ _lock = threading.Lock()
 
 def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> GLMProgram | None:
     """The wide-kernel program for ``spec``, or None when the model is not a
-    GLM or small enough for the register kernel."""
-    if spec.nparam < WIDE_MIN_PARAMS or spec.strict:
+    GLM, is small enough for the register kernel, or parks more LDS than a
+    workgroup may hold at this tile."""
+    if (spec.nparam < WIDE_MIN_PARAMS or spec.strict
+            or lds_bytes(spec.nparam, tile_max, spec.dtype) > LDS_LIMIT):
         return None
     key = (spec, tile_max)
     with _lock:

@@ -44,3 +44,36 @@ def test_identity_glm_with_offset():
 def test_exp_of_nonlinear_exponent_is_rejected():
     dag = hipgen.res_jac_hes("exp(a*b*t)", ["a", "b"], ["t"])
     assert glm.glm_form(dag) is None
+
+
+@pytest.mark.parametrize("n,tile", [(22, 136), (22, 100), (22, 170), (22, 40), (22, 10),
+                                    (11, 136), (9, 1)])
+def test_tile_rows_partition_the_packed_triangle(n, tile):
+    """Tiles are consecutive runs of whole rows, each as long as fits in
+    tile entries (a row longer than tile gets a tile of its own), covering
+    rows 0..n-1; the parked entries are those of all tiles but the last."""
+    from pycompute_amd import wide
+    rows = wide.tile_rows(n, tile)
+    assert rows[0] == 0 and rows[-1] == n and all(a < b for a, b in zip(rows, rows[1:]))
+    tri = lambda r: r * (r + 1) // 2
+    for r0, r1 in zip(rows, rows[1:]):
+        size = tri(r1) - tri(r0)
+        assert size <= tile or r1 == r0 + 1
+        if r1 < n:   # maximal: the next row would not fit
+            assert size + r1 + 1 > tile
+    assert wide.park_entries(n, tile) == tri(rows[-2])
+    if tile >= tri(n):
+        assert rows == [0, n] and wide.park_entries(n, tile) == 0
+
+
+def test_wide_kernel_lds_budget():
+    """The LDS park holds every tile but the last, one column per lane: the
+    22-parameter tiles in use fit one workgroup's 64 KB in FP32; FP64 at tile
+    136 does not, so the wide kernel declines (the register kernel runs)."""
+    from pycompute_amd import lm, wide
+    for tile in (40, 100, 136, 170):
+        assert wide.lds_bytes(22, tile, "float32") <= wide.LDS_LIMIT
+    assert wide.lds_bytes(22, 136, "float32") == 136 * 64 * 4
+    assert wide.lds_bytes(22, 136, "float64") > wide.LDS_LIMIT
+    spec = lm.LMSpec(DKI.expr, DKI.pars, DKI.consts, 61, "float64")
+    assert wide.glm_program(spec, 136) is None
