@@ -219,8 +219,9 @@ int pca_loglinear_init(const double* P, int32_t K, int32_t M, const void* y, int
 
 /* ---- learned initialiser (no counterpart in the reference; its initial
  * guesses are the segmented log-linear fit and random search,
- * test/ivim/test_ivim.ipynb cells 6 and 11-12). fp32 storage, bf16 MFMA
- * operands (v_mfma_f32_16x16x16_bf16), fp32 accumulate. */
+ * test/ivim/test_ivim.ipynb cells 6 and 11-12). fp32 storage and
+ * accumulation; training GEMMs take bf16 MFMA operands
+ * (v_mfma_f32_16x16x16_bf16), inference fp16 (v_mfma_f32_16x16x16_f16). */
 
 /* C (M x N) = op(A) . op(B): op(A)[m][k] = transA ? A[k*lda+m] : A[m*lda+k],
  * op(B)[k][n] = transB ? B[n*ldb+k] : B[k*ldb+n]. Epilogue epi: 0 store
@@ -243,9 +244,10 @@ int pca_mlp_colsum(const float* X, int64_t ld, int32_t M, int32_t N, float* out,
 int pca_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
                  float b2, float eps, int32_t step, float weight_decay, void* stream);
 
-/* p[0:n] = value; fp32 -> bf16 (round to nearest even). */
+/* p[0:n] = value; fp32 -> bf16, fp32 -> fp16 (round to nearest even). */
 int pca_mlp_fill(float* p, int64_t n, float value, void* stream);
 int pca_mlp_to_bf16(const float* x, int16_t* y, int64_t n, void* stream);
+int pca_mlp_to_f16(const float* x, int16_t* y, int64_t n, void* stream);
 
 /* Training inputs: X (in_padded, N) = y / y[norm_index] (zero rows past in),
  * T (N, out) = (transform(truth) - mu) / sd, the transform being: divide
@@ -260,14 +262,17 @@ int pca_mlp_prepare(const float* y, const float* truth, int64_t N, int32_t in, i
  * signals (in, N), writes clip(inverse transform(mu + sd * y), lb, ub) into
  * params (out, N) for voxels whose outputs are finite (others keep what
  * params held: the fallback guess). w1 is (hidden, in_padded) with
- * in_padded = 16, 32 or 64 and zero columns past `in`. */
+ * in_padded = 16, 32 or 64 and zero columns past `in`. Weights are fp16
+ * (pca_mlp_to_f16 of the fp32 masters); the activation is the tanh-form GELU
+ * evaluated as a packed-fp16 polynomial (absolute error <= 3e-4 in exact
+ * arithmetic, on top of fp16 rounding). */
 typedef struct pca_mlp_infer_args {
   const void* data;      /* (in, N) fp32 */
-  const int16_t* w1;     /* (hidden, in_padded) bf16 */
+  const int16_t* w1;     /* (hidden, in_padded) fp16 */
   const float* b1;       /* (hidden) */
-  const int16_t* w2;     /* (hidden, hidden) bf16 */
+  const int16_t* w2;     /* (hidden, hidden) fp16 */
   const float* b2;       /* (hidden) */
-  const int16_t* w3;     /* (out, hidden) bf16 */
+  const int16_t* w3;     /* (out, hidden) fp16 */
   const float* b3;       /* (out) */
   const float* mu;       /* (out) */
   const float* sd;       /* (out) */

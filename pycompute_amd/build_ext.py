@@ -18,7 +18,12 @@ LIB = os.path.join(PKG, "libpycompute_amd.so")
 ARCH = "gfx950"
 
 # Packed-FP32 VALU forms are kept out of every kernel built here.
-HIP_DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HIP_DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                    # MFMA sources and results in VGPRs rather than AGPRs: the
+                    # MLP kernels feed every accumulator straight into VALU work
+                    # (bias-seeded accumulators, activations), which the AGPR
+                    # form pays for with v_accvgpr_read copies and registers
+                    "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def hipcc() -> str:

@@ -121,6 +121,8 @@ k_loglinear_init(const double* __restrict__ P, int Kr, int M, const T* __restric
     for (int k = 0; k < KA; ++k)
       if (k < K) c[k] += P[k * M + i] * ly;
   }
+  // the host pairs kind 2 with KC = 3 and kind 3 with KC = 4 (or the generic
+  // size), so the size guards below only remove branches a launch never takes
   T p[KA];
   if (kind == 0) {
 #pragma unroll
@@ -129,16 +131,20 @@ k_loglinear_init(const double* __restrict__ P, int Kr, int M, const T* __restric
     p[0] = t_exp((T)c[0]);
     p[1] = (T)1 / (T)c[1];
   } else if (kind == 2) {
-    p[0] = t_exp((T)c[0]);
-    p[1] = (T)c[1];
-    p[2] = (T)1;
+    if constexpr (KA >= 3) {
+      p[0] = t_exp((T)c[0]);
+      p[1] = (T)c[1];
+      p[2] = (T)1;
+    }
   } else {
-    const T s_lo = t_exp((T)c[0]);
-    const T s_hi = t_exp((T)c[2]);
-    p[0] = s_lo;
-    p[1] = (T)1 - s_hi / s_lo;
-    p[2] = (T)c[1];
-    p[3] = (T)c[3];
+    if constexpr (KA >= 4) {
+      const T s_lo = t_exp((T)c[0]);
+      const T s_hi = t_exp((T)c[2]);
+      p[0] = s_lo;
+      p[1] = (T)1 - s_hi / s_lo;
+      p[2] = (T)c[1];
+      p[3] = (T)c[3];
+    }
   }
 #pragma unroll
   for (int j = 0; j < KA; ++j) {

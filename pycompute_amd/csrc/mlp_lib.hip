@@ -1,17 +1,20 @@
 // Learned initialiser: a small MLP (in -> H -> H -> out, GELU) mapping a
 // voxel's normalised signal to standardised starting parameters. Its dense
-// contractions run on the matrix cores: v_mfma_f32_16x16x16_bf16 (bf16
-// operands, fp32 accumulate). Compiled into libpycompute_amd.so for gfx950.
+// contractions run on the matrix cores with fp32 accumulation: training on
+// v_mfma_f32_16x16x16_bf16 (bf16 operands), inference on
+// v_mfma_f32_16x16x16_f16 (fp16 operands). Compiled into libpycompute_amd.so
+// for gfx950.
 //
 // There is no counterpart in the reference (its initial guesses are the
 // segmented log-linear fit and random search, test/ivim/test_ivim.ipynb
 // cells 6, 11-12); LM refines from the MLP output and the log-linear guess is
 // the fallback.
 //
-// Storage is fp32 (weights keep fp32 masters for Adam); operands are rounded
-// to bf16 (round to nearest even) as they are loaded into MFMA registers.
+// Storage is fp32 (weights keep fp32 masters for Adam); training operands
+// are rounded to bf16 (round to nearest even) as they are loaded into MFMA
+// registers; inference reads fp16 copies of the weights (pca_mlp_to_f16).
 //
-// MFMA 16x16x16 bf16 operand layout (wave64, lane l):
+// MFMA 16x16x16 (bf16 or f16) operand layout (wave64, lane l):
 //   A (16 x 16, M x K): row l % 16, k = 4 (l / 16) + 0..3
 //   B (16 x 16, K x N): col l % 16, k = 4 (l / 16) + 0..3
 //   C (16 x 16):        col l % 16, rows 4 (l / 16) + 0..3
@@ -54,6 +57,59 @@ __device__ __forceinline__ float gelu_grad(float x) {
   const float s = gelu_sigmoid(x);
   const float du = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
   return s + 2.0f * x * s * (1.0f - s) * du;
+}
+
+// Inference operands: fp16.
+typedef _Float16 half2 __attribute__((ext_vector_type(2)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma16h(half4 a, half4 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// The same tanh-form GELU as a packed-fp16 polynomial for inference:
+// gelu(x) = x s(x), s(x) = sigmoid(2u) = 1/2 + z P(z^2), z = clamp(x / 3.5,
+// -1, 1), P of degree 5, fitted to s on |x| <= 3.5 by iteratively reweighted
+// least squares on the error of x s(x) (a near-minimax fit). In exact
+// arithmetic |x s_P(x) - gelu(x)| <= 2.8e-4 for |x| <= 3.5; beyond, s_P is
+// held at its end value, within 2.6e-4 of 0 or 1. Two activations per
+// instruction (v_pk_mul_f16 / v_pk_fma_f16 / v_pk_max_f16 / v_pk_min_f16), no
+// transcendental instruction.
+constexpr float kGeluP[6] = {1.390737289680657f, -2.74187719999597f, 4.43028147414536f,
+                             -4.637580546145781f, 2.7523397819371955f, -0.694156068202283f};
+constexpr float kGeluClamp = 3.5f;
+__device__ __forceinline__ half2 splat2(float c) { return half2{(_Float16)c, (_Float16)c}; }
+__device__ __forceinline__ half2 gelu_pk(float a, float b) {
+  const half2 x = {(_Float16)a, (_Float16)b};
+  const half2 z = __builtin_elementwise_min(
+      __builtin_elementwise_max(x * splat2(1.0f / kGeluClamp), splat2(-1.0f)), splat2(1.0f));
+  const half2 t = z * z;
+  half2 p = t * splat2(kGeluP[5]) + splat2(kGeluP[4]);
+  p = p * t + splat2(kGeluP[3]);
+  p = p * t + splat2(kGeluP[2]);
+  p = p * t + splat2(kGeluP[1]);
+  p = p * t + splat2(kGeluP[0]);
+  return x * (z * p + splat2(0.5f));
+}
+__device__ __forceinline__ half4 gelu_pk4(floatx4 v) {
+  const half2 lo = gelu_pk(v[0], v[1]), hi = gelu_pk(v[2], v[3]);
+  return half4{lo.x, lo.y, hi.x, hi.y};
+}
+
+// Copy an (R x K) fp16 matrix (row-major, rows past rows_valid read as zero)
+// into LDS in MFMA fragment order: fragment f = (R/16 row tile, K/16 k tile)
+// holds 64 lanes x 4 values, lane l = 16 (k within tile / 4) + row within
+// tile, at dst[4 (64 f + l)].
+template <int R, int K>
+__device__ __forceinline__ void stage_fragments(short* dst, const short* src, int rows_valid) {
+  constexpr int NL = (R / 16) * (K / 16) * 64;
+  for (int i = threadIdx.x; i < NL; i += blockDim.x) {
+    const int f = i >> 6, l = i & 63;
+    const int row = (f / (K / 16)) * 16 + (l & 15), k = (f % (K / 16)) * 16 + (l >> 4) * 4;
+    long long v = 0;
+    if (row < rows_valid) v = *(const long long*)(src + (long long)row * K + k);
+    *(long long*)(dst + 4 * i) = v;
+  }
 }
 
 enum Epi { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_GRAD = 3 };
@@ -183,17 +239,23 @@ __global__ void k_to_bf16(const float* x, short* y, long long n) {
     y[i] = f2bf(x[i]);
 }
 
+__global__ void k_to_f16(const float* x, short* y, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = __builtin_bit_cast(short, (_Float16)x[i]);
+}
+
 // ---------------------------------------------------------------------------
 // Fused inference: normalise, three layers, de-standardise, clip.
 // ---------------------------------------------------------------------------
 
 struct InferArgs {
   const float* data;     // (IN, N) measurements, voxel fastest
-  const short* w1;       // (H, IN) bf16
+  const short* w1;       // (H, INP) fp16
   const float* b1;       // (H)
-  const short* w2;       // (H, H) bf16
+  const short* w2;       // (H, H) fp16
   const float* b2;       // (H)
-  const short* w3;       // (OUT, H) bf16
+  const short* w3;       // (OUT, H) fp16
   const float* b3;       // (OUT)
   const float* mu;       // (OUT) output de-standardisation: p = mu + sd * y
   const float* sd;       // (OUT)
@@ -216,22 +278,56 @@ struct InferArgs {
 // each weight fragment (A operand, 8 contiguous bytes per lane) is reused for
 // VT voxel tiles. Only the final OUT x 16VT outputs pass through LDS, to
 // gather each voxel's parameters into one lane.
-template <int H, int INP, int OUTP, int VT>
+//
+// Operands are fp16 (v_mfma_f32_16x16x16_f16), accumulation fp32. Each
+// accumulator starts from its bias, and the activation is the packed-fp16
+// polynomial GELU (gelu_pk): no transcendental instructions, two activations
+// per VALU instruction. With LDSW the three weight matrices are staged once
+// per workgroup into LDS in fragment order (fragment (row tile, k tile) = 64
+// lanes x 8 bytes, lane l at byte 8 l: conflict-free ds_read_b64), and the
+// grid is persistent.
+template <int H, int INP, int OUTP, int VT, bool LDSW>
 __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   constexpr int UT = H / 16;     // unit tiles per hidden layer
   constexpr int KT1 = INP / 16;  // k-steps of layer 1
   constexpr int OT = OUTP / 16;  // output tiles
   constexpr int V = 16 * VT;     // voxels per wavefront
+  constexpr int W2OFF = H * INP, W3OFF = H * INP + H * H;
   __shared__ float ybuf[4][OUTP * V];
+  __shared__ __attribute__((aligned(16))) short wl[LDSW ? W3OFF + OUTP * H : 4];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int col = lane & 15;      // voxel within a tile
   const int kq = (lane >> 4) * 4; // k (or row) offset within a tile
+  if constexpr (LDSW) {
+    stage_fragments<H, INP>(wl, a.w1, H);
+    stage_fragments<H, H>(wl + W2OFF, a.w2, H);
+    stage_fragments<OUTP, H>(wl + W3OFF, a.w3, a.OUT);
+    __syncthreads();
+  }
+  // weight fragment (row tile r, k tile k) of w1, w2, w3
+  auto w1f = [&](int r, int k) -> half4 {
+    if constexpr (LDSW) return *(const half4*)(wl + ((r * KT1 + k) * 64 + lane) * 4);
+    else return *(const half4*)(a.w1 + (long long)(r * 16 + col) * INP + k * 16 + kq);
+  };
+  auto w2f = [&](int r, int k) -> half4 {
+    if constexpr (LDSW) return *(const half4*)(wl + W2OFF + ((r * UT + k) * 64 + lane) * 4);
+    else return *(const half4*)(a.w2 + (long long)(r * 16 + col) * H + k * 16 + kq);
+  };
+  auto w3f = [&](int r, int k) -> half4 {
+    if constexpr (LDSW) {
+      return *(const half4*)(wl + W3OFF + ((r * UT + k) * 64 + lane) * 4);
+    } else {
+      half4 w = half4{0, 0, 0, 0};   // rows of W3 past OUT are zero
+      if (r * 16 + col < a.OUT) w = *(const half4*)(a.w3 + (long long)(r * 16 + col) * H + k * 16 + kq);
+      return w;
+    }
+  };
   float* yb = ybuf[wave];
   const long long nwaves = (long long)gridDim.x * 4;
   for (long long base = ((long long)blockIdx.x * 4 + wave) * V; base < a.N; base += nwaves * V) {
     // layer-1 B operands: normalised inputs x^T (k = feature, n = voxel)
-    bf16x4 xb[KT1][VT];
+    half4 xb[KT1][VT];
 #pragma unroll
     for (int t = 0; t < VT; ++t) {
       const long long v = base + t * 16 + col;
@@ -247,35 +343,43 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = kt * 16 + kq + q;
-          xb[kt][t][q] = f2bf((ok && i < a.IN) ? a.data[(long long)i * a.N + v] * inv : 0.f);
+          xb[kt][t][q] = (_Float16)((ok && i < a.IN) ? a.data[(long long)i * a.N + v] * inv : 0.f);
         }
     }
     // layer 1: h1^T = gelu(W1 x^T + b1)
-    bf16x4 h1[UT][VT];
+    half4 h1[UT][VT];
 #pragma unroll
     for (int u = 0; u < UT; ++u) {
+      floatx4 bias;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bias[q] = a.b1[u * 16 + kq + q];
       floatx4 acc[VT];
 #pragma unroll
-      for (int t = 0; t < VT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < VT; ++t) acc[t] = bias;
 #pragma unroll
       for (int kt = 0; kt < KT1; ++kt) {
-        const bf16x4 w = *(const bf16x4*)(a.w1 + (long long)(u * 16 + col) * INP + kt * 16 + kq);
+        const half4 w = w1f(u, kt);
 #pragma unroll
-        for (int t = 0; t < VT; ++t) acc[t] = mfma16(w, xb[kt][t], acc[t]);
+        for (int t = 0; t < VT; ++t) acc[t] = mfma16h(w, xb[kt][t], acc[t]);
       }
 #pragma unroll
-      for (int t = 0; t < VT; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) h1[u][t][q] = f2bf(gelu(acc[t][q] + a.b1[u * 16 + kq + q]));
+      for (int t = 0; t < VT; ++t) h1[u][t] = gelu_pk4(acc[t]);
     }
     // layers 2 and 3 fused: each 16-unit tile of h2^T = gelu(W2 h1^T + b2) is
-    // consumed at once as one k-step of y^T = W3 h2^T + b3 (rows of W3 past
-    // OUT are zero), so h2 never needs to be held whole
+    // consumed at once as one k-step of y^T = W3 h2^T + b3, so h2 never needs
+    // to be held whole
     floatx4 acc3[OT][VT];
 #pragma unroll
-    for (int o = 0; o < OT; ++o)
+    for (int o = 0; o < OT; ++o) {
+      floatx4 b3v;
 #pragma unroll
-      for (int t = 0; t < VT; ++t) acc3[o][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int q = 0; q < 4; ++q) {
+        const int j = o * 16 + kq + q;
+        b3v[q] = j < a.OUT ? a.b3[j] : 0.f;
+      }
+#pragma unroll
+      for (int t = 0; t < VT; ++t) acc3[o][t] = b3v;
+    }
     // rolled over pairs of tiles: u only selects weights and biases, so only
     // the weight fragments of two tiles are live at a time, and the two
     // tiles' accumulators give the matrix pipe 2 VT independent chains
@@ -283,36 +387,33 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
     for (int u0 = 0; u0 < UT; u0 += 2) {
       floatx4 acc[2][VT];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < 2; ++h) {
+        floatx4 bias;
 #pragma unroll
-        for (int t = 0; t < VT; ++t) acc[h][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int q = 0; q < 4; ++q) bias[q] = a.b2[(u0 + h) * 16 + kq + q];
+#pragma unroll
+        for (int t = 0; t < VT; ++t) acc[h][t] = bias;
+      }
 #pragma unroll
       for (int kt = 0; kt < UT; ++kt) {
-        bf16x4 w[2];
+        half4 w[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-          w[h] = *(const bf16x4*)(a.w2 + (long long)((u0 + h) * 16 + col) * H + kt * 16 + kq);
+        for (int h = 0; h < 2; ++h) w[h] = w2f(u0 + h, kt);
 #pragma unroll
         for (int t = 0; t < VT; ++t)
 #pragma unroll
-          for (int h = 0; h < 2; ++h) acc[h][t] = mfma16(w[h], h1[kt][t], acc[h][t]);
+          for (int h = 0; h < 2; ++h) acc[h][t] = mfma16h(w[h], h1[kt][t], acc[h][t]);
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int u = u0 + h;
-        bf16x4 h2u[VT];
+        half4 h2u[VT];
 #pragma unroll
-        for (int t = 0; t < VT; ++t)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            h2u[t][q] = f2bf(gelu(acc[h][t][q] + a.b2[u * 16 + kq + q]));
+        for (int t = 0; t < VT; ++t) h2u[t] = gelu_pk4(acc[h][t]);
 #pragma unroll
         for (int o = 0; o < OT; ++o) {
-          const int row = o * 16 + col;
-          bf16x4 w = bf16x4{0, 0, 0, 0};
-          if (row < a.OUT) w = *(const bf16x4*)(a.w3 + (long long)row * H + u * 16 + kq);
+          const half4 w = w3f(o, u0 + h);
 #pragma unroll
-          for (int t = 0; t < VT; ++t) acc3[o][t] = mfma16(w, h2u[t], acc3[o][t]);
+          for (int t = 0; t < VT; ++t) acc3[o][t] = mfma16h(w, h2u[t], acc3[o][t]);
         }
       }
     }
@@ -321,10 +422,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
 #pragma unroll
       for (int t = 0; t < VT; ++t)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int j = o * 16 + kq + q;
-          yb[j * V + t * 16 + col] = acc3[o][t][q] + (j < a.OUT ? a.b3[j] : 0.f);
-        }
+        for (int q = 0; q < 4; ++q) yb[(o * 16 + kq + q) * V + t * 16 + col] = acc3[o][t][q];
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     // one lane per voxel: inverse transform, finite check, clip, store
@@ -456,6 +554,13 @@ extern "C" int pca_mlp_to_bf16(const float* x, int16_t* y, int64_t n, void* stre
   return check_launch();
 }
 
+extern "C" int pca_mlp_to_f16(const float* x, int16_t* y, int64_t n, void* stream) {
+  if (!x || !y || n <= 0) return pca_fail(PCA_ERR_ARG, "pca_mlp_to_f16: bad argument");
+  hipLaunchKernelGGL(k_to_f16, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, x,
+                     (short*)y, (long long)n);
+  return check_launch();
+}
+
 extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
   if (!args || !args->data || !args->w1 || !args->w2 || !args->w3 || !args->params ||
       !args->lb || !args->ub || !args->mu || !args->sd || !args->b1 || !args->b2 || !args->b3)
@@ -475,24 +580,27 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
   const bool big = H == 256;
   const int V = big ? 32 : 64;  // voxels per wavefront
   const long long waves = (args->N + V - 1) / V;
-  const int grid = (int)std::min<long long>((waves + 3) / 4, 8192);
+  // weights in LDS when they fit next to the output buffer in 64 KB: then a
+  // persistent grid of 512 workgroups (2 per CU) reuses the staged copy
+  const bool ldsw = !big && OUTP == 16 && INP <= 32;
+  const int grid = (int)std::min<long long>((waves + 3) / 4, ldsw ? 512 : 8192);
   hipStream_t s = (hipStream_t)stream;
-#define PCA_INFER(HH, II, OO, VT) \
-  hipLaunchKernelGGL((k_mlp_infer_t<HH, II, OO, VT>), dim3(grid), dim3(256), 0, s, a)
+#define PCA_INFER(HH, II, OO, VT, L) \
+  hipLaunchKernelGGL((k_mlp_infer_t<HH, II, OO, VT, L>), dim3(grid), dim3(256), 0, s, a)
   if (!big) {
-    if (INP == 16 && OUTP == 16) PCA_INFER(128, 16, 16, 4);
-    else if (INP == 16) PCA_INFER(128, 16, 32, 4);
-    else if (INP == 32 && OUTP == 16) PCA_INFER(128, 32, 16, 4);
-    else if (INP == 32) PCA_INFER(128, 32, 32, 4);
-    else if (OUTP == 16) PCA_INFER(128, 64, 16, 4);
-    else PCA_INFER(128, 64, 32, 4);
+    if (INP == 16 && OUTP == 16) PCA_INFER(128, 16, 16, 4, true);
+    else if (INP == 16) PCA_INFER(128, 16, 32, 4, false);
+    else if (INP == 32 && OUTP == 16) PCA_INFER(128, 32, 16, 4, true);
+    else if (INP == 32) PCA_INFER(128, 32, 32, 4, false);
+    else if (OUTP == 16) PCA_INFER(128, 64, 16, 4, false);
+    else PCA_INFER(128, 64, 32, 4, false);
   } else {
-    if (INP == 16 && OUTP == 16) PCA_INFER(256, 16, 16, 2);
-    else if (INP == 16) PCA_INFER(256, 16, 32, 2);
-    else if (INP == 32 && OUTP == 16) PCA_INFER(256, 32, 16, 2);
-    else if (INP == 32) PCA_INFER(256, 32, 32, 2);
-    else if (OUTP == 16) PCA_INFER(256, 64, 16, 2);
-    else PCA_INFER(256, 64, 32, 2);
+    if (INP == 16 && OUTP == 16) PCA_INFER(256, 16, 16, 2, false);
+    else if (INP == 16) PCA_INFER(256, 16, 32, 2, false);
+    else if (INP == 32 && OUTP == 16) PCA_INFER(256, 32, 16, 2, false);
+    else if (INP == 32) PCA_INFER(256, 32, 32, 2, false);
+    else if (OUTP == 16) PCA_INFER(256, 64, 16, 2, false);
+    else PCA_INFER(256, 64, 32, 2, false);
   }
 #undef PCA_INFER
   return check_launch();

@@ -55,11 +55,13 @@ def _bind():
     L.pca_mlp_adam.argtypes = [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _F, _P]
     L.pca_mlp_fill.argtypes = [_P, _I64, _F, _P]
     L.pca_mlp_to_bf16.argtypes = [_P, _P, _I64, _P]
+    L.pca_mlp_to_f16.argtypes = [_P, _P, _I64, _P]
     L.pca_mlp_prepare.argtypes = [_P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P,
                                   _P, _P, _P]
     L.pca_mlp_infer.argtypes = [ctypes.POINTER(InferArgs), _P]
     for n in ("pca_mlp_gemm", "pca_mlp_mse_grad", "pca_mlp_colsum", "pca_mlp_adam",
-              "pca_mlp_fill", "pca_mlp_to_bf16", "pca_mlp_prepare", "pca_mlp_infer"):
+              "pca_mlp_fill", "pca_mlp_to_bf16", "pca_mlp_to_f16", "pca_mlp_prepare",
+              "pca_mlp_infer"):
         getattr(L, n).restype = ctypes.c_int
     L._mlp_bound = True
     return L
@@ -108,7 +110,7 @@ class MLPInit:
         self.offsets = {}
         off = 0
         for name, shp in shapes:
-            off = (off + 7) // 8 * 8   # 16-byte aligned blocks: bf16x4 weight loads
+            off = (off + 7) // 8 * 8   # 16-byte aligned blocks: 4-value weight loads
             self.offsets[name] = (off, shp)
             off += int(np.prod(shp))
         self.nparams = off
@@ -127,7 +129,7 @@ class MLPInit:
         self.g = torch.zeros(off, **kw)
         self.m = torch.zeros(off, **kw)
         self.v = torch.zeros(off, **kw)
-        self.wbf = torch.zeros(off, dtype=torch.int16, device=self.device)
+        self.wh = torch.zeros(off, dtype=torch.int16, device=self.device)   # fp16 bits
         self.mu = torch.zeros(O, **kw)
         self.sd = torch.ones(O, **kw)
         self.step = 0
@@ -283,17 +285,17 @@ class MLPInit:
         return self.history
 
     def finalize(self):
-        """bf16 copies of the weights for inference."""
+        """fp16 copies of the weights for inference."""
         L = _bind()
-        _lib.check(L.pca_mlp_to_bf16(self.w.data_ptr(), self.wbf.data_ptr(), self.nparams,
-                                     _stream()), "pca_mlp_to_bf16")
+        _lib.check(L.pca_mlp_to_f16(self.w.data_ptr(), self.wh.data_ptr(), self.nparams,
+                                    _stream()), "pca_mlp_to_f16")
 
     # -- inference ------------------------------------------------------------------
     def infer(self, data_t, lb_t, ub_t, params_t, stream=None):
         """params_t (out, N): fallback guess in, clipped MLP guess out."""
         L = _bind()
         N = int(data_t.shape[1])
-        wb = {k: self._view(self.wbf, k) for k in self.offsets}
+        wb = {k: self._view(self.wh, k) for k in self.offsets}
         wf = {k: self._view(self.w, k) for k in self.offsets}
         a = InferArgs(data_t.data_ptr(), wb["w1"].data_ptr(), wf["b1"].data_ptr(),
                       wb["w2"].data_ptr(), wf["b2"].data_ptr(), wb["w3"].data_ptr(),
