@@ -14,7 +14,8 @@
 // are rounded to bf16 (round to nearest even) as they are loaded into MFMA
 // registers; inference reads fp16 copies of the weights (pca_mlp_to_f16).
 //
-// MFMA 16x16x16 (bf16 or f16) operand layout (wave64, lane l):
+// MFMA 16x16x16 (bf16 or f16) operand layout (wave64, lane l; 16x16x32 as
+// noted at mfma32h):
 //   A (16 x 16, M x K): row l % 16, k = 4 (l / 16) + 0..3
 //   B (16 x 16, K x N): col l % 16, k = 4 (l / 16) + 0..3
 //   C (16 x 16):        col l % 16, rows 4 (l / 16) + 0..3
@@ -63,8 +64,17 @@ __device__ __forceinline__ float gelu_grad(float x) {
 typedef _Float16 half2 __attribute__((ext_vector_type(2)));
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
 __device__ __forceinline__ floatx4 mfma16h(half4 a, half4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+// 16x16x32 (gfx950): A row l % 16, B column l % 16, k = 8 (l / 16) + 0..7
+__device__ __forceinline__ floatx4 mfma32h(half8 a, half8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ half8 cat8(half4 a, half4 b) {
+  return half8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
 }
 
 // The same tanh-form GELU as a packed-fp16 polynomial for inference:
@@ -109,6 +119,26 @@ __device__ __forceinline__ void stage_fragments(short* dst, const short* src, in
     long long v = 0;
     if (row < rows_valid) v = *(const long long*)(src + (long long)row * K + k);
     *(long long*)(dst + 4 * i) = v;
+  }
+}
+
+// The same for 16x16x32 fragments with the k permutation of the inference
+// kernel: fragment f = (R/16 row tile, K/32 k block c), lane l = 16 g + row
+// within tile holds W[row][32 c + 4 g + 0..3] then W[row][32 c + 16 + 4 g +
+// 0..3], at dst[8 (64 f + l)].
+template <int R, int K>
+__device__ __forceinline__ void stage_fragments_k32(short* dst, const short* src, int rows_valid) {
+  constexpr int NL = (R / 16) * (K / 32) * 64;
+  for (int i = threadIdx.x; i < NL; i += blockDim.x) {
+    const int f = i >> 6, l = i & 63;
+    const int row = (f / (K / 32)) * 16 + (l & 15), k = (f % (K / 32)) * 32 + (l >> 4) * 4;
+    long long v0 = 0, v1 = 0;
+    if (row < rows_valid) {
+      v0 = *(const long long*)(src + (long long)row * K + k);
+      v1 = *(const long long*)(src + (long long)row * K + k + 16);
+    }
+    *(long long*)(dst + 8 * i) = v0;
+    *(long long*)(dst + 8 * i + 4) = v1;
   }
 }
 
@@ -272,80 +302,110 @@ struct InferArgs {
 
 // Register-resident inference in the transposed form: every layer computes
 // out^T (units x voxels) = W (units x k) . in^T (k x voxels). The MFMA result
-// layout (lane: column = voxel l%16, rows 4(l/16)+0..3 = units) is then
-// exactly the B-operand layout of the next layer (k = unit 4(l/16)+0..3,
-// column = voxel l%16), so activations stay in registers between layers, and
-// each weight fragment (A operand, 8 contiguous bytes per lane) is reused for
-// VT voxel tiles. Only the final OUT x 16VT outputs pass through LDS, to
-// gather each voxel's parameters into one lane.
+// layout (lane: column = voxel l%16, rows 4(l/16)+0..3 = units) is then the
+// B-operand layout of the next layer, so activations stay in registers
+// between layers, and each weight fragment (A operand) is reused for VT voxel
+// tiles. Only the final OUT x 16VT outputs pass through LDS, to gather each
+// voxel's parameters into one lane.
 //
-// Operands are fp16 (v_mfma_f32_16x16x16_f16), accumulation fp32. Each
-// accumulator starts from its bias, and the activation is the packed-fp16
-// polynomial GELU (gelu_pk): no transcendental instructions, two activations
-// per VALU instruction. With LDSW the three weight matrices are staged once
-// per workgroup into LDS in fragment order (fragment (row tile, k tile) = 64
-// lanes x 8 bytes, lane l at byte 8 l: conflict-free ds_read_b64), and the
-// grid is persistent.
+// Operands are fp16, accumulation fp32; every accumulator starts from its
+// bias, and the activation is the packed-fp16 polynomial GELU (gelu_pk).
+// Layer 1 uses v_mfma_f32_16x16x16_f16; layers 2 and 3 use
+// v_mfma_f32_16x16x32_f16, whose B operand (k = 8 (l/16) + 0..7) is the
+// concatenation of two consecutive 16-unit activation tiles: k slot 8g + i
+// holds unit 32c + 4g + i (i < 4) or 32c + 16 + 4g + i - 4 (i >= 4) of k
+// block c, and the weight fragments are gathered with the same permutation.
+// With LDSW the weights are staged once per workgroup into LDS in fragment
+// order (conflict-free 8- and 16-byte reads) and the grid is persistent.
 template <int H, int INP, int OUTP, int VT, bool LDSW>
 __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   constexpr int UT = H / 16;     // unit tiles per hidden layer
+  constexpr int KC = H / 32;     // k blocks of 32 units
   constexpr int KT1 = INP / 16;  // k-steps of layer 1
   constexpr int OT = OUTP / 16;  // output tiles
   constexpr int V = 16 * VT;     // voxels per wavefront
+  static_assert(V <= 64, "at most one voxel per lane in the epilogue");
   constexpr int W2OFF = H * INP, W3OFF = H * INP + H * H;
   __shared__ float ybuf[4][OUTP * V];
-  __shared__ __attribute__((aligned(16))) short wl[LDSW ? W3OFF + OUTP * H : 4];
+  __shared__ __attribute__((aligned(16))) short wl[LDSW ? W3OFF + OUTP * H : 8];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int col = lane & 15;      // voxel within a tile
   const int kq = (lane >> 4) * 4; // k (or row) offset within a tile
   if constexpr (LDSW) {
     stage_fragments<H, INP>(wl, a.w1, H);
-    stage_fragments<H, H>(wl + W2OFF, a.w2, H);
-    stage_fragments<OUTP, H>(wl + W3OFF, a.w3, a.OUT);
+    stage_fragments_k32<H, H>(wl + W2OFF, a.w2, H);
+    stage_fragments_k32<OUTP, H>(wl + W3OFF, a.w3, a.OUT);
     __syncthreads();
   }
-  // weight fragment (row tile r, k tile k) of w1, w2, w3
+  // weight fragments: w1 (row tile r, k tile k); w2, w3 (row tile r, k block c)
   auto w1f = [&](int r, int k) -> half4 {
     if constexpr (LDSW) return *(const half4*)(wl + ((r * KT1 + k) * 64 + lane) * 4);
     else return *(const half4*)(a.w1 + (long long)(r * 16 + col) * INP + k * 16 + kq);
   };
-  auto w2f = [&](int r, int k) -> half4 {
-    if constexpr (LDSW) return *(const half4*)(wl + W2OFF + ((r * UT + k) * 64 + lane) * 4);
-    else return *(const half4*)(a.w2 + (long long)(r * 16 + col) * H + k * 16 + kq);
-  };
-  auto w3f = [&](int r, int k) -> half4 {
+  auto w2f = [&](int r, int c) -> half8 {
     if constexpr (LDSW) {
-      return *(const half4*)(wl + W3OFF + ((r * UT + k) * 64 + lane) * 4);
+      return *(const half8*)(wl + W2OFF + ((r * KC + c) * 64 + lane) * 8);
     } else {
-      half4 w = half4{0, 0, 0, 0};   // rows of W3 past OUT are zero
-      if (r * 16 + col < a.OUT) w = *(const half4*)(a.w3 + (long long)(r * 16 + col) * H + k * 16 + kq);
+      const short* p = a.w2 + (long long)(r * 16 + col) * H + c * 32 + kq;
+      return cat8(*(const half4*)p, *(const half4*)(p + 16));
+    }
+  };
+  auto w3f = [&](int r, int c) -> half8 {
+    if constexpr (LDSW) {
+      return *(const half8*)(wl + W3OFF + ((r * KC + c) * 64 + lane) * 8);
+    } else {
+      half8 w = half8{0, 0, 0, 0, 0, 0, 0, 0};   // rows of W3 past OUT are zero
+      if (r * 16 + col < a.OUT) {
+        const short* p = a.w3 + (long long)(r * 16 + col) * H + c * 32 + kq;
+        w = cat8(*(const half4*)p, *(const half4*)(p + 16));
+      }
       return w;
     }
   };
   float* yb = ybuf[wave];
   const long long nwaves = (long long)gridDim.x * 4;
   for (long long base = ((long long)blockIdx.x * 4 + wave) * V; base < a.N; base += nwaves * V) {
-    // layer-1 B operands: normalised inputs x^T (k = feature, n = voxel)
+    // layer-1 B operands: normalised inputs x^T (k = feature, n = voxel); the
+    // normaliser of tile t stays in sv[t] for the epilogue
     half4 xb[KT1][VT];
+    float sv[VT];
 #pragma unroll
     for (int t = 0; t < VT; ++t) {
       const long long v = base + t * 16 + col;
       const bool ok = v < a.N;
-      float s = 1.f;
-      if (ok) {
-        s = a.data[(long long)a.norm_index * a.N + v];
-        if (!(s > 0.f)) s = 1.f;
-      }
-      const float inv = 1.f / s;
+      float s = ok ? a.data[(long long)a.norm_index * a.N + v] : 1.f;
+      float xr[KT1][4];
 #pragma unroll
       for (int kt = 0; kt < KT1; ++kt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = kt * 16 + kq + q;
-          xb[kt][t][q] = (_Float16)((ok && i < a.IN) ? a.data[(long long)i * a.N + v] * inv : 0.f);
+          xr[kt][q] = (ok && i < a.IN) ? a.data[(long long)i * a.N + v] : 0.f;
         }
+      if (!(s > 0.f)) s = 1.f;
+      sv[t] = s;
+      const float inv = 1.f / s;
+#pragma unroll
+      for (int kt = 0; kt < KT1; ++kt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xb[kt][t][q] = (_Float16)(xr[kt][q] * inv);
     }
+    // epilogue operands, loaded now so their latency hides under the layers:
+    // lane l finishes voxel base + l
+    const long long ve = base + lane;
+    const bool mine = lane < V && ve < a.N;
+    float lo[OUTP], hi[OUTP];
+#pragma unroll
+    for (int j = 0; j < OUTP; ++j)
+      if (j < a.OUT && mine) {
+        lo[j] = a.lb[(long long)j * a.N + ve];
+        hi[j] = a.ub[(long long)j * a.N + ve];
+      }
+    float se = sv[0];
+#pragma unroll
+    for (int t = 1; t < VT; ++t)
+      if ((lane >> 4) == t) se = sv[t];
     // layer 1: h1^T = gelu(W1 x^T + b1)
     half4 h1[UT][VT];
 #pragma unroll
@@ -365,9 +425,9 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
 #pragma unroll
       for (int t = 0; t < VT; ++t) h1[u][t] = gelu_pk4(acc[t]);
     }
-    // layers 2 and 3 fused: each 16-unit tile of h2^T = gelu(W2 h1^T + b2) is
-    // consumed at once as one k-step of y^T = W3 h2^T + b3, so h2 never needs
-    // to be held whole
+    // layers 2 and 3 fused: each pair of 16-unit tiles of h2^T = gelu(W2 h1^T
+    // + b2) is consumed at once as one k block of y^T = W3 h2^T + b3, so h2
+    // never needs to be held whole
     floatx4 acc3[OT][VT];
 #pragma unroll
     for (int o = 0; o < OT; ++o) {
@@ -380,41 +440,42 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
 #pragma unroll
       for (int t = 0; t < VT; ++t) acc3[o][t] = b3v;
     }
-    // rolled over pairs of tiles: u only selects weights and biases, so only
-    // the weight fragments of two tiles are live at a time, and the two
-    // tiles' accumulators give the matrix pipe 2 VT independent chains
+    // rolled over k blocks of h2: only the weight fragments of two unit tiles
+    // are live at a time, and their accumulators give the matrix pipe 2 VT
+    // independent chains
 #pragma unroll 1
-    for (int u0 = 0; u0 < UT; u0 += 2) {
+    for (int c = 0; c < KC; ++c) {
       floatx4 acc[2][VT];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         floatx4 bias;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) bias[q] = a.b2[(u0 + h) * 16 + kq + q];
+        for (int q = 0; q < 4; ++q) bias[q] = a.b2[(2 * c + h) * 16 + kq + q];
 #pragma unroll
         for (int t = 0; t < VT; ++t) acc[h][t] = bias;
       }
 #pragma unroll
-      for (int kt = 0; kt < UT; ++kt) {
-        half4 w[2];
+      for (int kc = 0; kc < KC; ++kc) {
+        half8 w[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) w[h] = w2f(u0 + h, kt);
+        for (int h = 0; h < 2; ++h) w[h] = w2f(2 * c + h, kc);
 #pragma unroll
-        for (int t = 0; t < VT; ++t)
+        for (int t = 0; t < VT; ++t) {
+          const half8 b = cat8(h1[2 * kc][t], h1[2 * kc + 1][t]);
 #pragma unroll
-          for (int h = 0; h < 2; ++h) acc[h][t] = mfma16h(w[h], h1[kt][t], acc[h][t]);
-      }
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        half4 h2u[VT];
-#pragma unroll
-        for (int t = 0; t < VT; ++t) h2u[t] = gelu_pk4(acc[h][t]);
-#pragma unroll
-        for (int o = 0; o < OT; ++o) {
-          const half4 w = w3f(o, u0 + h);
-#pragma unroll
-          for (int t = 0; t < VT; ++t) acc3[o][t] = mfma16h(w, h2u[t], acc3[o][t]);
+          for (int h = 0; h < 2; ++h) acc[h][t] = mfma32h(w[h], b, acc[h][t]);
         }
+      }
+      half4 h2[2][VT];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < VT; ++t) h2[h][t] = gelu_pk4(acc[h][t]);
+#pragma unroll
+      for (int o = 0; o < OT; ++o) {
+        const half8 w = w3f(o, c);
+#pragma unroll
+        for (int t = 0; t < VT; ++t) acc3[o][t] = mfma32h(w, cat8(h2[0][t], h2[1][t]), acc3[o][t]);
       }
     }
 #pragma unroll
@@ -426,17 +487,14 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     // one lane per voxel: inverse transform, finite check, clip, store
-    for (int vv = lane; vv < V; vv += 64) {
-      const long long v = base + vv;
-      if (v >= a.N) continue;
-      float s = a.data[(long long)a.norm_index * a.N + v];
-      if (!(s > 0.f)) s = 1.f;
+    if (mine) {
+      const float s = se;
       bool finite = true;
       float p[OUTP];
 #pragma unroll
       for (int j = 0; j < OUTP; ++j) {
         if (j < a.OUT) {
-          float t = a.mu[j] + a.sd[j] * yb[j * V + vv];
+          float t = a.mu[j] + a.sd[j] * yb[j * V + lane];
           if ((a.log_mask >> j) & 1) t = expf(t);
           if (j == a.scale_param) t *= s;
           if (j == a.logshift_param) t += logf(s);
@@ -448,10 +506,9 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
 #pragma unroll
         for (int j = 0; j < OUTP; ++j) {
           if (j < a.OUT) {
-            const float lo = a.lb[(long long)j * a.N + v], hi = a.ub[(long long)j * a.N + v];
-            float t = p[j] < lo ? lo : p[j];
-            t = t > hi ? hi : t;
-            a.params[(long long)j * a.N + v] = t;
+            float t = p[j] < lo[j] ? lo[j] : p[j];
+            t = t > hi[j] ? hi[j] : t;
+            a.params[(long long)j * a.N + ve] = t;
           }
         }
       }
