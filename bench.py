@@ -131,8 +131,8 @@ def run_mlp(args, prob, knobs, plain_ssr, stream):
     log(f"mlp trained on {args.train_samples} signals in {train_s:.2f} s, loss {hist}")
 
     def mlp_guess(p, pr):
-        fit.default_init(pr, out=p)                       # fallback for non-finite outputs
-        net.infer(pr.data_t, pr.lb_t, pr.ub_t, p)
+        # voxels with non-finite outputs get the default guess in the same kernel
+        net.infer(pr.data_t, pr.lb_t, pr.ub_t, p, fallback=(pr.model, pr.consts))
 
     # validation volume: choose the iteration cap
     wv = synth.ivim(1 << 18, synth.B_C2, seed=99, snr=(20.0, 100.0))
@@ -175,9 +175,8 @@ def run_mlp(args, prob, knobs, plain_ssr, stream):
     solver.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
 
     def init(st):
-        fit._init.default_init(prob.model, prob.consts, prob.data_t, prob.lb_t, prob.ub_t,
-                               out=pars, stream=st)
-        net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, stream=st)
+        net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, stream=st,
+                  fallback=(prob.model, prob.consts))
 
     solver.capture_graph(cap, init=init)
     for _ in range(args.warmup):
@@ -194,7 +193,7 @@ def run_mlp(args, prob, knobs, plain_ssr, stream):
     # component times: MLP inference alone
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars)
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, fallback=(prob.model, prob.consts))
     e1.record(stream)
     torch.cuda.synchronize()
     infer_ms = e0.elapsed_time(e1)
@@ -343,10 +342,12 @@ def run_ivim_4m(args, knobs, net, cap):
         s.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
 
         def init(st, pars=pars, use_mlp=use_mlp):
-            fit._init.default_init(prob.model, prob.consts, prob.data_t, prob.lb_t, prob.ub_t,
-                                   out=pars, stream=st)
             if use_mlp:
-                net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, stream=st)
+                net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, stream=st,
+                          fallback=(prob.model, prob.consts))
+            else:
+                fit._init.default_init(prob.model, prob.consts, prob.data_t, prob.lb_t,
+                                       prob.ub_t, out=pars, stream=st)
 
         s.capture_graph(iters, init=init)
         ms = _time_replays(s.replay_graph, args)

@@ -283,6 +283,12 @@ typedef struct pca_mlp_infer_args {
   int32_t in, out, hidden;
   int32_t norm_index, scale_param, logshift_param, log_mask;
   int32_t reserved;
+  /* optional fallback for voxels whose outputs are not finite: the
+   * log-linear guess of pca_loglinear_init (P: K x in, double, on the device;
+   * kind as there, with n = out), nan_to_num'd and clipped. NULL: such voxels
+   * keep what params held. */
+  const double* fallback_P;
+  int32_t fallback_K, fallback_kind;
 } pca_mlp_infer_args;
 int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream);
 

@@ -103,6 +103,18 @@ def wide_candidates(allow_fast: bool = True):
     return out
 
 
+def pick(results: list) -> dict:
+    """The winning entry of timed candidates ({"knobs", "ms", "agree"}, in
+    candidate order). Rounding-level variants must agree with the
+    reference-rounding build (the first candidate) on nearly every voxel --
+    FP32 and FP64 runs of the reference itself differ on about 1% -- and
+    timings within 1% of the fastest are a tie (run-to-run noise is of that
+    order), won by the earliest candidate so that repeated tunings agree."""
+    ok = [r for r in results if r["agree"] >= 0.97]
+    fastest = min(r["ms"] for r in ok)
+    return next(r for r in ok if r["ms"] <= fastest * 1.01)
+
+
 def tune(make_solver, setup, model_expr: str, M: int, dtype: str, allow_fast: bool = True,
          iters: int = 100, reps: int = 3, force: bool = False, log=None,
          cands=None) -> dict:
@@ -142,10 +154,7 @@ def tune(make_solver, setup, model_expr: str, M: int, dtype: str, allow_fast: bo
         results.append({"knobs": cand, "ms": ms, "agree": agree})
         if log:
             log(f"autotune {cand} {ms:.3f} ms agree {agree:.4f}")
-    # rounding-level variants must agree with the reference-rounding build on
-    # nearly every voxel (FP32 vs FP64 of the reference itself differs on ~1%)
-    ok = [r for r in results if r["agree"] >= 0.97]
-    best = min(ok, key=lambda r: r["ms"])
+    best = pick(results)
     db[k] = {"best": best["knobs"], "ms": best["ms"], "results": results,
              "when": time.strftime("%Y-%m-%d %H:%M:%S")}
     try:

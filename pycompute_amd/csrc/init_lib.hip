@@ -26,6 +26,7 @@
 
 #include "pycompute_amd.h"
 #include "internal.h"
+#include "loglinear.h"
 
 namespace {
 
@@ -82,24 +83,6 @@ k_bpolyfit(const T* __restrict__ x, int x_per_voxel, const T* __restrict__ y, in
   for (int a = 0; a < K; ++a) out[(long long)a * N + v] = (T)r[a];
 }
 
-template <typename T>
-__device__ __forceinline__ T nan_to_num(T a) {
-  return (__builtin_isnan(a) || __builtin_isinf(a)) ? (T)0 : a;
-}
-
-template <typename T>
-__device__ __forceinline__ T t_log(T a);
-template <>
-__device__ __forceinline__ float t_log<float>(float a) { return logf(a); }
-template <>
-__device__ __forceinline__ double t_log<double>(double a) { return log(a); }
-template <typename T>
-__device__ __forceinline__ T t_exp(T a);
-template <>
-__device__ __forceinline__ float t_exp<float>(float a) { return expf(a); }
-template <>
-__device__ __forceinline__ double t_exp<double>(double a) { return exp(a); }
-
 // KC: compile-time size of the per-voxel arrays, max(rows of P, outputs)
 // (2, 3 or 4 for the built-in models; 0 = generic, up to kMaxK). The rows of
 // P actually read are Kr.
@@ -111,45 +94,12 @@ k_loglinear_init(const double* __restrict__ P, int Kr, int M, const T* __restric
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
   constexpr int KA = KC > 0 ? KC : kMaxK;
-  const int K = Kr;
-  double c[KA];
-#pragma unroll
-  for (int k = 0; k < KA; ++k) c[k] = 0.0;
-  for (int i = 0; i < M; ++i) {
-    const double ly = (double)t_log(y[(long long)i * N + v]);
-#pragma unroll
-    for (int k = 0; k < KA; ++k)
-      if (k < K) c[k] += P[k * M + i] * ly;
-  }
-  // the host pairs kind 2 with KC = 3 and kind 3 with KC = 4 (or the generic
-  // size), so the size guards below only remove branches a launch never takes
   T p[KA];
-  if (kind == 0) {
-#pragma unroll
-    for (int k = 0; k < KA; ++k) p[k] = (T)c[k];
-  } else if (kind == 1) {
-    p[0] = t_exp((T)c[0]);
-    p[1] = (T)1 / (T)c[1];
-  } else if (kind == 2) {
-    if constexpr (KA >= 3) {
-      p[0] = t_exp((T)c[0]);
-      p[1] = (T)c[1];
-      p[2] = (T)1;
-    }
-  } else {
-    if constexpr (KA >= 4) {
-      const T s_lo = t_exp((T)c[0]);
-      const T s_hi = t_exp((T)c[2]);
-      p[0] = s_lo;
-      p[1] = (T)1 - s_hi / s_lo;
-      p[2] = (T)c[1];
-      p[3] = (T)c[3];
-    }
-  }
+  pca_init::loglinear_guess<T, KA>(P, Kr, M, y, N, v, kind, p);
 #pragma unroll
   for (int j = 0; j < KA; ++j) {
     if (j < n) {
-      T a = nan_to_num(p[j]);
+      T a = pca_init::nan_to_num(p[j]);
       const T lo = lb[(long long)j * N + v], hi = ub[(long long)j * N + v];
       a = a < lo ? lo : a;
       a = a > hi ? hi : a;

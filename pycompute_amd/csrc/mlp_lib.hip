@@ -26,6 +26,7 @@
 #include <cstdint>
 
 #include "internal.h"
+#include "loglinear.h"
 #include "pycompute_amd.h"
 
 namespace {
@@ -298,6 +299,8 @@ struct InferArgs {
   int scale_param;       // parameter predicted relative to s (p *= s), -1: none
   int logshift_param;    // parameter predicted as a log relative to s (p += log s), -1: none
   int log_mask;          // bit j: parameter j is predicted as log(p_j)
+  const double* fb_P;    // fallback log-linear guess (loglinear.h) for voxels whose
+  int fb_K, fb_kind;     //   outputs are not finite; null: params keep what they held
 };
 
 // Register-resident inference in the transposed form: every layer computes
@@ -502,6 +505,14 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
           p[j] = t;
         }
       }
+      if (!finite && a.fb_P) {
+        // rare: the log-linear guess instead, nan_to_num as pca_loglinear_init
+        pca_init::loglinear_guess<float, OUTP>(a.fb_P, a.fb_K, a.IN, a.data, a.N, ve,
+                                               a.fb_kind, p);
+#pragma unroll
+        for (int j = 0; j < OUTP; ++j) p[j] = pca_init::nan_to_num(p[j]);
+        finite = true;
+      }
       if (finite) {
 #pragma unroll
         for (int j = 0; j < OUTP; ++j) {
@@ -626,11 +637,19 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
   if (IN < 1 || IN > 64 || OUT < 1 || OUT > 32 || (H != 128 && H != 256) || args->N < 0 ||
       args->norm_index < 0 || args->norm_index >= IN)
     return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: sizes must be in <= 64, out <= 32, hidden 128|256");
+  if (args->fallback_P) {
+    const int K = args->fallback_K, kind = args->fallback_kind;
+    if ((kind == 0 && K != OUT) || (kind == 1 && (K != 2 || OUT != 2)) ||
+        (kind == 2 && (K != 2 || OUT != 3)) || (kind == 3 && (K != 4 || OUT != 4)) ||
+        kind < 0 || kind > 3)
+      return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: fallback K/kind do not match the outputs");
+  }
   if (args->N == 0) return PCA_OK;
   InferArgs a{(const float*)args->data, (const short*)args->w1, args->b1, (const short*)args->w2,
               args->b2, (const short*)args->w3, args->b3, args->mu, args->sd,
               (const float*)args->lb, (const float*)args->ub, (float*)args->params, args->N, IN,
-              OUT, args->norm_index, args->scale_param, args->logshift_param, args->log_mask};
+              OUT, args->norm_index, args->scale_param, args->logshift_param, args->log_mask,
+              args->fallback_P, args->fallback_K, args->fallback_kind};
   // the input width is padded to a multiple of 16 and stored with that stride
   const int INP = IN <= 16 ? 16 : (IN <= 32 ? 32 : 64);
   const int OUTP = OUT <= 16 ? 16 : 32;

@@ -67,16 +67,23 @@ def init_matrix(model: ModelSpec, consts: np.ndarray) -> np.ndarray:
 _P_CACHE: dict = {}
 
 
-def default_init(model: ModelSpec, consts, data_t, lb_t, ub_t, out=None, stream=None):
-    """Initial parameters (n, N) for ``data_t`` (M, N) on its device."""
+def loglinear_spec(model: ModelSpec, consts, device):
+    """(P, kind) of the model's default guess: P (K, M) float64 on ``device``
+    (cached per model, protocol and device), kind as pca_loglinear_init."""
     if model.name not in KIND:
         raise ValueError(f"no default init for model {model.name!r}")
     consts = np.asarray(consts, np.float64)
-    key = (model.name, consts.tobytes(), str(data_t.device))
+    key = (model.name, consts.tobytes(), str(device))
     P = _P_CACHE.get(key)
     if P is None:
-        P = torch.tensor(init_matrix(model, consts), dtype=torch.float64, device=data_t.device)
+        P = torch.tensor(init_matrix(model, consts), dtype=torch.float64, device=device)
         _P_CACHE[key] = P
+    return P, KIND[model.name]
+
+
+def default_init(model: ModelSpec, consts, data_t, lb_t, ub_t, out=None, stream=None):
+    """Initial parameters (n, N) for ``data_t`` (M, N) on its device."""
+    P, _ = loglinear_spec(model, consts, data_t.device)
     K, M = P.shape
     n = model.nparam
     N = int(data_t.shape[1])
@@ -100,7 +107,8 @@ def default_init(model: ModelSpec, consts, data_t, lb_t, ub_t, out=None, stream=
     return out
 
 
-__all__ = ["default_init", "init_matrix", "ivim_split", "IVIM", "T2", "SE", "DKI"]
+__all__ = ["default_init", "loglinear_spec", "init_matrix", "ivim_split", "IVIM", "T2", "SE",
+           "DKI"]
 
 
 class PickArgs(ctypes.Structure):

@@ -9,14 +9,15 @@ the generated module's pca_synth; torch only holds the buffers):
     net.fit_standardization(truth, data)
     X, T = net.prepare(data, truth)
     net.train(X, T, epochs=2, batch=8192)
-    net.infer(data_t, lb_t, ub_t, params_t)     # params_t: fallback in, MLP guess out
+    net.infer(data_t, lb_t, ub_t, params_t, fallback=(model, consts))  # MLP guess out
 
 The network sees the signal normalised by its first measurement (b = 0 for
 the diffusion protocols) and predicts standardised, transformed parameters:
 a parameter relative to that signal level (S0), log-parameters for the rates
 spanning decades, the rest as is. LM then refines from the prediction; voxels
-whose prediction is not finite keep the fallback (the default log-linear
-guess). Training: mean squared error on the standardised targets, Adam.
+whose prediction is not finite get the default log-linear guess, computed in
+the inference kernel itself. Training: mean squared error on the standardised
+targets, Adam.
 """
 from __future__ import annotations
 
@@ -41,7 +42,8 @@ class InferArgs(ctypes.Structure):
                 ("b3", _P), ("mu", _P), ("sd", _P), ("lb", _P), ("ub", _P), ("params", _P),
                 ("N", _I64), ("in_", _I32), ("out", _I32), ("hidden", _I32),
                 ("norm_index", _I32), ("scale_param", _I32), ("logshift_param", _I32),
-                ("log_mask", _I32), ("reserved", _I32)]
+                ("log_mask", _I32), ("reserved", _I32), ("fallback_P", _P),
+                ("fallback_K", _I32), ("fallback_kind", _I32)]
 
 
 def _bind():
@@ -291,10 +293,21 @@ class MLPInit:
                                     _stream()), "pca_mlp_to_f16")
 
     # -- inference ------------------------------------------------------------------
-    def infer(self, data_t, lb_t, ub_t, params_t, stream=None):
-        """params_t (out, N): fallback guess in, clipped MLP guess out."""
+    def infer(self, data_t, lb_t, ub_t, params_t, stream=None, fallback=None):
+        """params_t (out, N): the clipped MLP guess out. Voxels whose outputs
+        are not finite get the log-linear guess of ``fallback`` = (model,
+        consts), computed in the same kernel; with fallback None they keep
+        what params_t held."""
         L = _bind()
         N = int(data_t.shape[1])
+        fP, fK, fkind = None, 0, 0
+        if fallback is not None:
+            from . import init as _init
+            P, fkind = _init.loglinear_spec(fallback[0], fallback[1], data_t.device)
+            if P.shape[1] != self.IN:
+                raise ValueError(f"fallback guess is for {P.shape[1]} measurements, the "
+                                 f"network takes {self.IN}")
+            fP, fK = P.data_ptr(), int(P.shape[0])
         wb = {k: self._view(self.wh, k) for k in self.offsets}
         wf = {k: self._view(self.w, k) for k in self.offsets}
         a = InferArgs(data_t.data_ptr(), wb["w1"].data_ptr(), wf["b1"].data_ptr(),
@@ -302,7 +315,7 @@ class MLPInit:
                       wf["b3"].data_ptr(), self.mu.data_ptr(), self.sd.data_ptr(),
                       lb_t.data_ptr(), ub_t.data_ptr(), params_t.data_ptr(), N, self.IN,
                       self.OUT, self.H, self.norm_index, self.scale_param, self.logshift_param,
-                      self.log_mask, 0)
+                      self.log_mask, 0, fP, fK, fkind)
         self._infer_args = a  # kept alive for graph capture
         _lib.check(L.pca_mlp_infer(ctypes.byref(a), _stream() if stream is None else stream),
                    "pca_mlp_infer")

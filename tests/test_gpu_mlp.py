@@ -9,7 +9,7 @@ import torch
 from oracle import mlp as omlp
 from pycompute_amd import fit, synth
 from pycompute_amd import mlp
-from pycompute_amd.models import IVIM
+from pycompute_amd.models import IVIM, T2
 
 pytestmark = pytest.mark.gpu
 
@@ -108,6 +108,43 @@ def test_fused_inference_matches_oracle_forward(gpu):
     rel = np.abs(got - p) / np.abs(p)
     print("inference rel err p99", np.percentile(rel, 99), "max", rel.max(), flush=True)
     assert np.percentile(rel, 99) < 2e-2 and np.median(rel) < 2e-3
+
+
+def test_inference_fallback_is_the_default_guess(gpu):
+    """Voxels whose network outputs are not finite get the log-linear default
+    guess from the same kernel, identical to pca_loglinear_init; without a
+    fallback they keep what params held."""
+    net = _small_net(seed=1)
+    w = synth.ivim_c2(3000, seed=9)
+    prob = fit.make_problem("ivim", w["consts"], w["data"])
+    net.finalize()
+    ref = fit.default_init(prob)
+    # every output non-finite
+    net.sd.fill_(float("inf"))
+    p = torch.full((4, prob.N), -7.0, device="cuda")
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, p, fallback=(prob.model, prob.consts))
+    q = torch.full((4, prob.N), -7.0, device="cuda")
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, q)
+    torch.cuda.synchronize()
+    assert torch.equal(p, ref)
+    assert bool((q == -7.0).all())
+    # only the voxels with a NaN sample: the rest keep the network's guess
+    net.sd.fill_(1.0)
+    data = prob.data_t.clone()
+    bad = torch.arange(0, prob.N, 7, device="cuda")
+    data[3, bad] = float("nan")
+    r = torch.full((4, prob.N), -7.0, device="cuda")
+    net.infer(data, prob.lb_t, prob.ub_t, r, fallback=(prob.model, prob.consts))
+    good = torch.full((4, prob.N), -7.0, device="cuda")
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, good)
+    dref = fit._init.default_init(prob.model, prob.consts, data, prob.lb_t, prob.ub_t)
+    torch.cuda.synchronize()
+    mask = torch.zeros(prob.N, dtype=torch.bool, device="cuda")
+    mask[bad] = True
+    assert torch.equal(r[:, mask], dref[:, mask])
+    assert torch.equal(r[:, ~mask], good[:, ~mask])
+    with pytest.raises(Exception):
+        net.infer(prob.data_t, prob.lb_t, prob.ub_t, r, fallback=(T2, [[10.0] * 16]))
 
 
 def test_synth_matches_model(gpu):

@@ -169,3 +169,14 @@ def test_code_gen_walking_links_post_order_once():
     code = code_gen_walking(Fn("top", [b, c]))
     order = [s for s in code.split() if s.startswith("/*")]
     assert order == ["/*a*/", "/*b*/", "/*c*/", "/*top*/"]
+
+
+def test_autotune_pick_ties_and_agreement():
+    from pycompute_amd import autotune
+    r = [{"knobs": {"k": 0}, "ms": 4.0, "agree": 1.0},
+         {"knobs": {"k": 1}, "ms": 2.010, "agree": 0.99},
+         {"knobs": {"k": 2}, "ms": 2.000, "agree": 0.99},
+         {"knobs": {"k": 3}, "ms": 1.0, "agree": 0.5}]     # fast but disagrees
+    assert autotune.pick(r)["knobs"] == {"k": 1}           # 0.5% slower: a tie, earlier wins
+    r[1]["ms"] = 2.05
+    assert autotune.pick(r)["knobs"] == {"k": 2}
