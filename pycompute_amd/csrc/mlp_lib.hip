@@ -62,7 +62,6 @@ __device__ __forceinline__ float gelu_grad(float x) {
 }
 
 // Inference operands: fp16.
-typedef _Float16 half2 __attribute__((ext_vector_type(2)));
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -89,22 +88,23 @@ __device__ __forceinline__ half8 cat8(half4 a, half4 b) {
 constexpr float kGeluP[6] = {1.390737289680657f, -2.74187719999597f, 4.43028147414536f,
                              -4.637580546145781f, 2.7523397819371955f, -0.694156068202283f};
 constexpr float kGeluClamp = 3.5f;
-__device__ __forceinline__ half2 splat2(float c) { return half2{(_Float16)c, (_Float16)c}; }
-__device__ __forceinline__ half2 gelu_pk(float a, float b) {
-  const half2 x = {(_Float16)a, (_Float16)b};
-  const half2 z = __builtin_elementwise_min(
-      __builtin_elementwise_max(x * splat2(1.0f / kGeluClamp), splat2(-1.0f)), splat2(1.0f));
-  const half2 t = z * z;
-  half2 p = t * splat2(kGeluP[5]) + splat2(kGeluP[4]);
-  p = p * t + splat2(kGeluP[3]);
-  p = p * t + splat2(kGeluP[2]);
-  p = p * t + splat2(kGeluP[1]);
-  p = p * t + splat2(kGeluP[0]);
-  return x * (z * p + splat2(0.5f));
+// Four activations as one half4 expression: every step is two independent
+// packed instructions, so neither waits on the other's result.
+__device__ __forceinline__ half4 splat4(float c) {
+  const _Float16 h = (_Float16)c;
+  return half4{h, h, h, h};
 }
 __device__ __forceinline__ half4 gelu_pk4(floatx4 v) {
-  const half2 lo = gelu_pk(v[0], v[1]), hi = gelu_pk(v[2], v[3]);
-  return half4{lo.x, lo.y, hi.x, hi.y};
+  const half4 x = {(_Float16)v[0], (_Float16)v[1], (_Float16)v[2], (_Float16)v[3]};
+  const half4 z = __builtin_elementwise_min(
+      __builtin_elementwise_max(x * splat4(1.0f / kGeluClamp), splat4(-1.0f)), splat4(1.0f));
+  const half4 t = z * z;
+  half4 p = t * splat4(kGeluP[5]) + splat4(kGeluP[4]);
+  p = p * t + splat4(kGeluP[3]);
+  p = p * t + splat4(kGeluP[2]);
+  p = p * t + splat4(kGeluP[1]);
+  p = p * t + splat4(kGeluP[0]);
+  return x * (z * p + splat4(0.5f));
 }
 
 // Copy an (R x K) fp16 matrix (row-major, rows past rows_valid read as zero)
@@ -320,8 +320,10 @@ struct InferArgs {
 // block c, and the weight fragments are gathered with the same permutation.
 // With LDSW the weights are staged once per workgroup into LDS in fragment
 // order (conflict-free 8- and 16-byte reads) and the grid is persistent.
-template <int H, int INP, int OUTP, int VT, bool LDSW>
+template <int H, int INP, int OUTP, int VT, bool LDSW, int OUTC>
 __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
+  // outputs: OUTC when fixed at compile time (the per-output guards fold away), else a.OUT
+  const int nout = OUTC > 0 ? OUTC : a.OUT;
   constexpr int UT = H / 16;     // unit tiles per hidden layer
   constexpr int KC = H / 32;     // k blocks of 32 units
   constexpr int KT1 = INP / 16;  // k-steps of layer 1
@@ -338,7 +340,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   if constexpr (LDSW) {
     stage_fragments<H, INP>(wl, a.w1, H);
     stage_fragments_k32<H, H>(wl + W2OFF, a.w2, H);
-    stage_fragments_k32<OUTP, H>(wl + W3OFF, a.w3, a.OUT);
+    stage_fragments_k32<OUTP, H>(wl + W3OFF, a.w3, nout);
     __syncthreads();
   }
   // weight fragments: w1 (row tile r, k tile k); w2, w3 (row tile r, k block c)
@@ -359,7 +361,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
       return *(const half8*)(wl + W3OFF + ((r * KC + c) * 64 + lane) * 8);
     } else {
       half8 w = half8{0, 0, 0, 0, 0, 0, 0, 0};   // rows of W3 past OUT are zero
-      if (r * 16 + col < a.OUT) {
+      if (r * 16 + col < nout) {
         const short* p = a.w3 + (long long)(r * 16 + col) * H + c * 32 + kq;
         w = cat8(*(const half4*)p, *(const half4*)(p + 16));
       }
@@ -401,7 +403,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
     float lo[OUTP], hi[OUTP];
 #pragma unroll
     for (int j = 0; j < OUTP; ++j)
-      if (j < a.OUT && mine) {
+      if (j < nout && mine) {
         lo[j] = a.lb[(long long)j * a.N + ve];
         hi[j] = a.ub[(long long)j * a.N + ve];
       }
@@ -438,7 +440,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int j = o * 16 + kq + q;
-        b3v[q] = j < a.OUT ? a.b3[j] : 0.f;
+        b3v[q] = j < nout ? a.b3[j] : 0.f;
       }
 #pragma unroll
       for (int t = 0; t < VT; ++t) acc3[o][t] = b3v;
@@ -496,7 +498,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
       float p[OUTP];
 #pragma unroll
       for (int j = 0; j < OUTP; ++j) {
-        if (j < a.OUT) {
+        if (j < nout) {
           float t = a.mu[j] + a.sd[j] * yb[j * V + lane];
           if ((a.log_mask >> j) & 1) t = expf(t);
           if (j == a.scale_param) t *= s;
@@ -516,7 +518,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
       if (finite) {
 #pragma unroll
         for (int j = 0; j < OUTP; ++j) {
-          if (j < a.OUT) {
+          if (j < nout) {
             float t = p[j] < lo[j] ? lo[j] : p[j];
             t = t > hi[j] ? hi[j] : t;
             a.params[(long long)j * a.N + ve] = t;
@@ -661,23 +663,33 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
   const bool ldsw = !big && OUTP == 16 && INP <= 32;
   const int grid = (int)std::min<long long>((waves + 3) / 4, ldsw ? 512 : 8192);
   hipStream_t s = (hipStream_t)stream;
-#define PCA_INFER(HH, II, OO, VT, L) \
-  hipLaunchKernelGGL((k_mlp_infer_t<HH, II, OO, VT, L>), dim3(grid), dim3(256), 0, s, a)
+#define PCA_INFER(HH, II, OO, VT, L, OC) \
+  hipLaunchKernelGGL((k_mlp_infer_t<HH, II, OO, VT, L, OC>), dim3(grid), dim3(256), 0, s, a)
+  // the small-output networks of the built-in models get their output count
+  // at compile time
+#define PCA_INFER_OUT(HH, II, OO, VT, L)                                  \
+  do {                                                                    \
+    if (OUT == 2) PCA_INFER(HH, II, OO, VT, L, 2);                        \
+    else if (OUT == 3) PCA_INFER(HH, II, OO, VT, L, 3);                   \
+    else if (OUT == 4) PCA_INFER(HH, II, OO, VT, L, 4);                   \
+    else PCA_INFER(HH, II, OO, VT, L, 0);                                 \
+  } while (0)
   if (!big) {
-    if (INP == 16 && OUTP == 16) PCA_INFER(128, 16, 16, 4, true);
-    else if (INP == 16) PCA_INFER(128, 16, 32, 4, false);
-    else if (INP == 32 && OUTP == 16) PCA_INFER(128, 32, 16, 4, true);
-    else if (INP == 32) PCA_INFER(128, 32, 32, 4, false);
-    else if (OUTP == 16) PCA_INFER(128, 64, 16, 4, false);
-    else PCA_INFER(128, 64, 32, 4, false);
+    if (INP == 16 && OUTP == 16) PCA_INFER_OUT(128, 16, 16, 4, true);
+    else if (INP == 16) PCA_INFER(128, 16, 32, 4, false, 0);
+    else if (INP == 32 && OUTP == 16) PCA_INFER_OUT(128, 32, 16, 4, true);
+    else if (INP == 32) PCA_INFER(128, 32, 32, 4, false, 0);
+    else if (OUTP == 16) PCA_INFER(128, 64, 16, 4, false, 0);
+    else PCA_INFER(128, 64, 32, 4, false, 0);
   } else {
-    if (INP == 16 && OUTP == 16) PCA_INFER(256, 16, 16, 2, false);
-    else if (INP == 16) PCA_INFER(256, 16, 32, 2, false);
-    else if (INP == 32 && OUTP == 16) PCA_INFER(256, 32, 16, 2, false);
-    else if (INP == 32) PCA_INFER(256, 32, 32, 2, false);
-    else if (OUTP == 16) PCA_INFER(256, 64, 16, 2, false);
-    else PCA_INFER(256, 64, 32, 2, false);
+    if (INP == 16 && OUTP == 16) PCA_INFER(256, 16, 16, 2, false, 0);
+    else if (INP == 16) PCA_INFER(256, 16, 32, 2, false, 0);
+    else if (INP == 32 && OUTP == 16) PCA_INFER(256, 32, 16, 2, false, 0);
+    else if (INP == 32) PCA_INFER(256, 32, 32, 2, false, 0);
+    else if (OUTP == 16) PCA_INFER(256, 64, 16, 2, false, 0);
+    else PCA_INFER(256, 64, 32, 2, false, 0);
   }
+#undef PCA_INFER_OUT
 #undef PCA_INFER
   return check_launch();
 }
