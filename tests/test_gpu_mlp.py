@@ -1,7 +1,8 @@
 """GPU checks of the learned initialiser: the bf16-MFMA GEMM against torch,
 training gradients against the CPU oracle (oracle/mlp.py, bf16 rounding at the
-same points), fused inference against the oracle forward, the synthetic
-signal generator against the model, and what the trained network gains."""
+same points), the fp16 fused inference kernels against the float64 oracle
+forward, the in-kernel fallback, the synthetic signal generator against the
+model, and what the trained network gains."""
 import numpy as np
 import pytest
 import torch
@@ -9,7 +10,7 @@ import torch
 from oracle import mlp as omlp
 from pycompute_amd import fit, synth
 from pycompute_amd import mlp
-from pycompute_amd.models import IVIM, T2
+from pycompute_amd.models import IVIM, T2, ModelSpec
 
 pytestmark = pytest.mark.gpu
 
@@ -99,7 +100,8 @@ def test_fused_inference_matches_oracle_forward(gpu):
     X = np.zeros((net.INP, prob.N))
     X[:net.IN] = y / s
     wt = {k: net._view(net.w, k).double().cpu().numpy() for k in net.offsets}
-    out = omlp.forward(X.T, wt["w1"], wt["b1"], wt["w2"], wt["b2"], wt["w3"], wt["b3"])
+    out = omlp.forward(X.T, wt["w1"], wt["b1"], wt["w2"], wt["b2"], wt["w3"], wt["b3"],
+                       round_bf16=False)
     mu, sd = net.mu.double().cpu().numpy(), net.sd.double().cpu().numpy()
     p = mu + sd * out
     p[:, 2:] = np.exp(p[:, 2:])
@@ -108,6 +110,57 @@ def test_fused_inference_matches_oracle_forward(gpu):
     rel = np.abs(got - p) / np.abs(p)
     print("inference rel err p99", np.percentile(rel, 99), "max", rel.max(), flush=True)
     assert np.percentile(rel, 99) < 2e-2 and np.median(rel) < 2e-3
+
+
+def _plain_net(model, consts, hidden, seed):
+    """A network without output transforms (mu 0, sd 1): inference returns
+    the raw forward pass, clipped to [-1e30, 1e30]."""
+    return mlp.MLPInit(model, consts, hidden=hidden, norm_index=0, seed=seed)
+
+
+_FAKE6 = ModelSpec("fake6", "a0", ("a0", "a1", "a2", "a3", "a4", "a5"), ("b",), (0,) * 6,
+                   (1,) * 6)
+
+
+@pytest.mark.parametrize("case", ["ivim128", "ivim256", "dki128", "dki256", "six128"])
+def test_inference_variants_match_oracle_forward(gpu, case):
+    """Every instantiation family of the fused inference kernel against the
+    float64 forward pass of the same weights: LDS-staged weights with the
+    output count fixed (IVIM, H=128) or at run time (six outputs), and
+    weights from global memory (H=256; 61 inputs)."""
+    if case.startswith("dki"):
+        w = synth.dki(2000, seed=3)
+        model = fit.MODELS["dki"]
+    else:
+        w = synth.ivim_c2(2000, seed=9)
+        model = _FAKE6 if case == "six128" else IVIM
+    hidden = 256 if case.endswith("256") else 128
+    net = _plain_net(model, w["consts"], hidden, seed=4)
+    net.finalize()
+    y = np.asarray(w["data"], np.float64)
+    N = y.shape[1]
+    data_t = torch.tensor(w["data"], dtype=torch.float32, device="cuda")
+    out = torch.zeros((net.OUT, N), device="cuda")
+    big = torch.full_like(out, 1e30)
+    net.infer(data_t, -big, big, out)
+    torch.cuda.synchronize()
+    X = np.zeros((net.INP, N))
+    X[:net.IN] = y / y[0]
+    wt = {k: net._view(net.w, k).double().cpu().numpy() for k in net.offsets}
+    ref = omlp.forward(X.T, wt["w1"], wt["b1"], wt["w2"], wt["b2"], wt["w3"], wt["b3"],
+                       round_bf16=False)
+    got = out.double().cpu().numpy().T
+    scale = np.abs(ref) + 0.1 * np.sqrt(np.mean(ref ** 2))
+    rel = np.abs(got - ref) / scale
+    # yardstick: the same forward pass with bf16-rounded operands and the
+    # exact GELU (the oracle's model of bf16 MFMA inference)
+    bf = omlp.forward(X.T, wt["w1"], wt["b1"], wt["w2"], wt["b2"], wt["w3"], wt["b3"])
+    rel_bf = np.abs(bf - ref) / scale
+    print(case, "inference err p99", np.percentile(rel, 99), "median", np.median(rel),
+          "| bf16 operands p99", np.percentile(rel_bf, 99), "median", np.median(rel_bf),
+          flush=True)
+    assert np.percentile(rel, 99) <= np.percentile(rel_bf, 99)
+    assert np.median(rel) <= np.median(rel_bf) and np.percentile(rel, 99) < 5e-2
 
 
 def test_inference_fallback_is_the_default_guess(gpu):
