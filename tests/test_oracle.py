@@ -238,7 +238,8 @@ def test_resume_equals_single_run():
 
 def test_float32_restatement_tracks_fp64_oracle():
     """On a well-determined model the FP32 restatement converges to the FP64
-    optimum (error against the FP64 oracle, stated per config in DESIGN.md)."""
+    optimum (the error against the FP64 oracle is what the bench reports per
+    configuration)."""
     from pycompute_amd import synth
     w = synth.t2(2048, seed=3, snr=200.0)
     lb, ub = _bounds("t2", 2048)
