@@ -64,3 +64,26 @@ def test_argument_errors_without_gpu():
     a.data = a.lb = a.ub = a.params = a.lam = a.step_type = a.f = 1
     rc = L.pca_lm_launch(None, ctypes.byref(a), big, ctypes.sizeof(big), 256, None)
     assert rc == -4 and b"PCA_UC_MAX" in L.pca_last_error()
+
+
+def test_mlp_infer_args_layout_and_validation():
+    """pca_mlp_infer_args as the header lays it out (the optional fallback
+    fields after `reserved`), and argument errors caught before any launch."""
+    from pycompute_amd import mlp
+    A = mlp.InferArgs
+    assert A.N.offset == 12 * 8 and A.reserved.offset == 12 * 8 + 8 + 7 * 4
+    assert A.fallback_P.offset == 136 and A.fallback_kind.offset == 148
+    assert ctypes.sizeof(A) == 152
+    L = mlp._bind()
+    a = A(1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1000, 16, 4, 128, 0, 0, -1, 12, 0,
+          None, 0, 0)
+    a.hidden = 100                                   # not 128 or 256
+    assert L.pca_mlp_infer(ctypes.byref(a), None) == -1
+    assert b"hidden" in L.pca_last_error()
+    a.hidden = 128
+    a.fallback_P, a.fallback_K, a.fallback_kind = 1, 2, 1   # T2 guess for a 4-output net
+    assert L.pca_mlp_infer(ctypes.byref(a), None) == -1
+    assert b"fallback" in L.pca_last_error()
+    a.w1 = None
+    assert L.pca_mlp_infer(ctypes.byref(a), None) == -1
+    assert b"NULL" in L.pca_last_error()
