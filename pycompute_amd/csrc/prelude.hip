@@ -22,12 +22,23 @@ typedef double real;
 // reproducible implementations of strict_math.h (concatenated before this
 // file) and is compiled with -ffp-contract=off, so the CPU oracle can replay
 // it bit for bit. The fast-exp build uses v_exp_f32 on x*log2(e) for float
-// (relative error about |x| * 6e-8).
+// (relative error about |x| * 6e-8), and v_log_f32 for log and pow.
 __device__ __forceinline__ float pca_exp(float x) { return expf(x); }
 __device__ __forceinline__ double pca_exp(double x) { return exp(x); }
 __device__ __forceinline__ float pca_exp_fast(float x) { return __expf(x); }
 __device__ __forceinline__ double pca_exp_fast(double x) { return exp(x); }
 __device__ __forceinline__ float pca_log(float x) { return logf(x); }
+// Fast-exp build: v_log_f32 (log2) for log, and pow as 2^(y log2 x) with
+// pow(x, 0) = 1; for float, errors of a few ulp times |y log2 x|. Double
+// keeps the library functions.
+__device__ __forceinline__ float pca_log_fast(float x) {
+  return __builtin_amdgcn_logf(x) * 0.69314718055994531f;
+}
+__device__ __forceinline__ double pca_log_fast(double x) { return log(x); }
+__device__ __forceinline__ float pca_pow_fast(float x, float y) {
+  return y == 0.0f ? 1.0f : __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+__device__ __forceinline__ double pca_pow_fast(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ double pca_log(double x) { return log(x); }
 __device__ __forceinline__ float pca_pow(float x, float y) { return powf(x, y); }
 __device__ __forceinline__ double pca_pow(double x, double y) { return pow(x, y); }
@@ -38,8 +49,8 @@ __device__ __forceinline__ double pca_pow(double x, double y) { return pow(x, y)
 #define PCA_POW(x, y) pca_pow_strict(x, y)
 #elif defined(PCA_FAST_EXP)
 #define PCA_EXP(x) pca_exp_fast(x)
-#define PCA_LOG(x) pca_log(x)
-#define PCA_POW(x, y) pca_pow(x, y)
+#define PCA_LOG(x) pca_log_fast(x)
+#define PCA_POW(x, y) pca_pow_fast(x, y)
 #else
 #define PCA_EXP(x) pca_exp(x)
 #define PCA_LOG(x) pca_log(x)

@@ -39,7 +39,10 @@ def make_problem(model, consts, data, device="cuda", dtype=torch.float32, lb=Non
     consts = np.asarray(consts, np.float64)
     if consts.ndim == 1:
         consts = consts[None, :]
-    data_t = torch.as_tensor(np.ascontiguousarray(data), device=device).to(dtype).contiguous()
+    if isinstance(data, torch.Tensor):
+        data_t = data.to(device=device, dtype=dtype).contiguous()
+    else:
+        data_t = torch.as_tensor(np.ascontiguousarray(data), device=device).to(dtype).contiguous()
     N = data_t.shape[1]
     lb = np.asarray(model.lb if lb is None else lb, np.float64)
     ub = np.asarray(model.ub if ub is None else ub, np.float64)

@@ -126,6 +126,10 @@ def test_default_init_matches_oracle(gpu):
     rel = _rel(p0, ref, 1e-6)
     _summary("init rel", rel.max(axis=0))
     assert np.mean(rel.max(axis=0) < 1e-4) >= 0.999
+    # the same problem built from a tensor already on the GPU
+    prob_t = fit.make_problem("ivim", w["consts"], torch.as_tensor(w["data"], device="cuda"))
+    assert torch.equal(prob_t.data_t, prob.data_t)
+    assert np.array_equal(fit.default_init(prob_t).cpu().numpy(), p0)
 
 
 def test_resume_equals_single_run(gpu):
