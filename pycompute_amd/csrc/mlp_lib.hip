@@ -37,7 +37,6 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ short f2bf(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
-
 __device__ __forceinline__ floatx4 mfma16(bf16x4 a, bf16x4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
