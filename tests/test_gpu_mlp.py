@@ -257,3 +257,36 @@ def test_trained_network_gains_over_default_init(gpu):
     best = [c for c, (d, it) in res.items() if d <= 1e-4]
     assert best, res
     assert res[min(best)][1] * 2.0 < it_plain
+
+
+def test_mlp_initialised_graph_replays_equal_eager(gpu):
+    """The headline pipeline -- state reset, inference with the fused
+    fallback, capped chunked LM -- captured as one hipGraph and replayed
+    several times with eager work in between equals the eager sequence on
+    every replay."""
+    net = _small_net(seed=3)
+    w = synth.ivim(16384, synth.B_C2, seed=5, snr=(20.0, 100.0))
+    prob = fit.make_problem("ivim", w["consts"], w["data"])
+    net.mu.copy_(torch.tensor([0.05, 0.2, -3.5, -6.5]))
+    net.sd.copy_(torch.tensor([0.1, 0.1, 0.5, 0.5]))
+    net.finalize()
+    fb = (prob.model, prob.consts)
+    p0 = torch.empty((4, prob.N), device="cuda")
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, p0, fallback=fb)
+    ref = fit.fit(prob, iters=6, params0=p0.clone(), compaction=True, chunk=24)
+    torch.cuda.synchronize()
+    pars = torch.empty_like(p0)
+    s = fit.make_solver(prob, compaction=True, chunk=24)
+    s.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+    s.capture_graph(6, init=lambda st: net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars,
+                                                 stream=st, fallback=fb))
+    for rep in range(3):
+        pars.fill_(float("nan"))
+        s.lam_t.fill_(123.0)
+        junk = torch.randn(1 << 20, device="cuda").sum()   # eager work between replays
+        torch.cuda.synchronize()
+        s.replay_graph()
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(junk))
+        for k in ("pars_t", "step_type_t", "lam_t", "f_t", "n_iter_t"):
+            assert torch.equal(getattr(ref, k), getattr(s, k)), (rep, k)
