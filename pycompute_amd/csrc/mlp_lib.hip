@@ -81,8 +81,7 @@ __device__ __forceinline__ half8 cat8(half4 a, half4 b) {
 // least squares on the error of x s(x) (a near-minimax fit). In exact
 // arithmetic |x s_P(x) - gelu(x)| <= 2.8e-4 for |x| <= 3.5; beyond, s_P is
 // held at its end value, within 2.6e-4 of 0 or 1. Two activations per
-// instruction (v_pk_mul_f16 / v_pk_fma_f16 / v_pk_max_f16 / v_pk_min_f16), no
-// transcendental instruction.
+// instruction (v_pk_mul_f16 / v_pk_fma_f16), no transcendental instruction.
 constexpr float kGeluP[6] = {1.390737289680657f, -2.74187719999597f, 4.43028147414536f,
                              -4.637580546145781f, 2.7523397819371955f, -0.694156068202283f};
 constexpr float kGeluClamp = 3.5f;
@@ -94,8 +93,12 @@ __device__ __forceinline__ half4 splat4(float c) {
 }
 __device__ __forceinline__ half4 gelu_pk4(floatx4 v) {
   const half4 x = {(_Float16)v[0], (_Float16)v[1], (_Float16)v[2], (_Float16)v[3]};
-  const half4 z = __builtin_elementwise_min(
-      __builtin_elementwise_max(x * splat4(1.0f / kGeluClamp), splat4(-1.0f)), splat4(1.0f));
+  // z = clamp(x / c, -1, 1) as 2w - 1 with w = clamp(x / 2c + 1/2, 0, 1): w is
+  // one v_pk_fma_f16 with the output clamp, so z costs two instructions
+  const half4 w = __builtin_elementwise_min(
+      __builtin_elementwise_max(x * splat4(0.5f / kGeluClamp) + splat4(0.5f), splat4(0.0f)),
+      splat4(1.0f));
+  const half4 z = w * splat4(2.0f) - splat4(1.0f);
   const half4 t = z * z;
   half4 p = t * splat4(kGeluP[5]) + splat4(kGeluP[4]);
   p = p * t + splat4(kGeluP[3]);
