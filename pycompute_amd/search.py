@@ -150,37 +150,76 @@ class SecondOrderRandomSearch(_SearchBase):
     """Multi-start LM (lsqnonlin.py:1245-1291): each iteration starts the LM
     from a random point in the bounds and keeps the result with the lowest
     final SSR. Knobs of the LM (compaction, chunk, fast_exp, ...) are passed
-    to SecondOrderLevenbergMarquardt."""
+    to SecondOrderLevenbergMarquardt.
+
+    Up to `batch` starts run as one LM over batch x N virtual voxels (the
+    voxels' data, consts and bounds repeated, start k in columns [k N,
+    (k + 1) N)), so one compacting launch sequence serves them all instead of
+    one per start. Per-voxel LM runs do not interact, and the starts are kept
+    in order, so results are identical to running the starts one by one
+    (batch=1). That pays off for small volumes, where one start cannot fill
+    the GPU; batch=None (the default) batches up to about 2^20 virtual voxels,
+    at most 16 starts."""
 
     def __init__(self, *args, seed: int = 0, block: int = lm.DEFAULT_BLOCK,
-                 strict: bool = False, **kwargs):
+                 strict: bool = False, batch: int | None = None, **kwargs):
         lm_kw = {k: kwargs.pop(k) for k in list(kwargs)
                  if k in ("compaction", "chunk", "fast_exp", "fast_div", "reassoc", "unroll",
                           "waves_per_eu", "wide")}
         super().__init__(*args, seed=seed, block=block, strict=strict, **kwargs)
-        self.lm = SecondOrderLevenbergMarquardt(self.expr, list(self.pars_str),
-                                                list(self.consts_str), self.ndata, self.dtype,
-                                                block=block, keep_workspace=False,
-                                                strict=strict, **lm_kw)
+        if batch is not None and int(batch) < 1:
+            raise ValueError("batch must be at least 1")
+        self.batch = None if batch is None else int(batch)
+        self._lm_args = dict(block=block, keep_workspace=False, strict=strict, **lm_kw)
+        self._virtual = None     # (K, solver, cand_v, views) of the last batch size
 
     def setup(self, consts_t, data_t, lower_bound_t, upper_bound_t):
         super().setup(consts_t, data_t, lower_bound_t, upper_bound_t)
         self.cand_t = torch.empty((self.nparam, self.Nprobs), dtype=self.dtype,
                                   device=self.data_t.device)
-        self.lm.setup(self.cand_t, self.consts_t, self.data_t, self.lower_bound_t,
-                      self.upper_bound_t)
+        self._virtual = None
+
+    def _batch_solver(self, K: int):
+        """The LM over K x N virtual voxels (built once per batch size)."""
+        if self._virtual is not None and self._virtual[0] == K:
+            return self._virtual
+        N = self.Nprobs
+        lm_ = SecondOrderLevenbergMarquardt(self.expr, list(self.pars_str),
+                                            list(self.consts_str), self.ndata, self.dtype,
+                                            **self._lm_args)
+        data_v = self.data_t.repeat(1, K)
+        lb_v = self.lower_bound_t.repeat(1, K)
+        ub_v = self.upper_bound_t.repeat(1, K)
+        if self._cpv is None:                 # voxel-uniform: one protocol for all
+            consts_v = torch.tensor(self._uc, dtype=self.dtype, device=self.data_t.device)
+        else:
+            consts_v = self._cpv.reshape(self.nconst * self.ndata, N).repeat(1, K)
+        cand_v = torch.empty((self.nparam, K * N), dtype=self.dtype, device=self.data_t.device)
+        lm_.setup(cand_v, consts_v, data_v, lb_v, ub_v)
+        self._virtual = (K, lm_, cand_v, (data_v, lb_v, ub_v, consts_v))
+        return self._virtual
 
     def run(self, iters: int, lm_iters: int = 20, tol=1e-8):
         if not hasattr(self, "data_t"):
             raise RuntimeError("setup() must be called before run()")
-        for _ in range(int(iters)):
-            a = RParamsArgs(self.lower_bound_t.data_ptr(), self.upper_bound_t.data_ptr(),
-                            self.cand_t.data_ptr(), self.seed, self.Nprobs, self.draws)
-            self._launch("pca_random_params", a, with_uc=False)
-            self.lm.reset_state()
-            self.lm.run(int(lm_iters), tol)
-            k = KeepArgs(self.cand_t.data_ptr(), self.data_t.data_ptr(), lm.ptr(self._cpv),
-                         self.best_pars_t.data_ptr(), self.best_f_t.data_ptr(), self.Nprobs)
-            self._launch("pca_keep_best", k)
-            self.draws += 1
+        left = int(iters)
+        N = self.Nprobs
+        most = self.batch if self.batch is not None else min(16, max(1, -(-(1 << 20) // N)))
+        while left > 0:
+            K = min(left, most)
+            _, solver, cand_v, _ = self._batch_solver(K)
+            for k in range(K):
+                a = RParamsArgs(self.lower_bound_t.data_ptr(), self.upper_bound_t.data_ptr(),
+                                self.cand_t.data_ptr(), self.seed, N, self.draws + k)
+                self._launch("pca_random_params", a, with_uc=False)
+                cand_v[:, k * N:(k + 1) * N].copy_(self.cand_t)
+            solver.reset_state()
+            solver.run(int(lm_iters), tol)
+            for k in range(K):
+                self.cand_t.copy_(cand_v[:, k * N:(k + 1) * N])
+                kp = KeepArgs(self.cand_t.data_ptr(), self.data_t.data_ptr(), lm.ptr(self._cpv),
+                              self.best_pars_t.data_ptr(), self.best_f_t.data_ptr(), N)
+                self._launch("pca_keep_best", kp)
+            self.draws += K
+            left -= K
         return None

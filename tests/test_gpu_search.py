@@ -107,3 +107,26 @@ def test_second_order_random_search_default_build(gpu):
     f0 = np.stack([oracle.ssr("ivim", prob.consts, np.ascontiguousarray(w["data"]),
                               np.ascontiguousarray(c[k])) for k in range(3)]).min(0)
     assert np.all(bf <= f0 * (1 + 1e-5))
+
+
+@pytest.mark.parametrize("per_voxel", [False, True])
+def test_batched_multistart_equals_one_start_at_a_time(gpu, per_voxel):
+    """Starts batched into one LM over K x N virtual voxels give the same best
+    parameters and SSR, bit for bit, as the starts run one by one; runs
+    continue the stream either way (5 starts = a batch of 3, then 2)."""
+    w, prob = _problem(3000, 23)
+    consts = prob.consts_t
+    if per_voxel:                               # b-values 0.01% apart on even voxels
+        consts = consts[:, None].expand(-1, prob.N).contiguous()
+        consts[:, ::2] *= 1.0001
+    out = {}
+    for batch in (1, 3):
+        srs = SecondOrderRandomSearch(IVIM.expr, list(IVIM.pars), list(IVIM.consts), prob.M,
+                                      torch.float32, seed=7, batch=batch)
+        srs.setup(consts, prob.data_t, prob.lb_t, prob.ub_t)
+        srs.run(2, lm_iters=15)
+        srs.run(3, lm_iters=15)
+        torch.cuda.synchronize()
+        out[batch] = (srs.best_pars_t.clone(), srs.best_f_t.clone(), srs.draws)
+    assert torch.equal(out[1][0], out[3][0]) and torch.equal(out[1][1], out[3][1])
+    assert out[1][2] == out[3][2] == 5
