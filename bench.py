@@ -489,11 +489,13 @@ def run_dki(args, stream):
                      "total_final_ssr": ssr, "total_ssr_rel_vs_plain": ssr / plain_ssr - 1.0,
                      "matched_total_residual": bool(ssr <= plain_ssr * (1 + 1e-4))}
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
     net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars)
+    e0.record(stream)
+    for _ in range(5):
+        net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars)
     e1.record(stream)
     torch.cuda.synchronize()
-    infer_ms = e0.elapsed_time(e1)
+    infer_ms = e0.elapsed_time(e1) / 5
     mlp_flops = 2.0 * N * (61 * 256 + 256 * 256 + 256 * 22)
     res["mlp_lm"].update({"train_samples": args.dki_train_samples,
                           "train_epochs": args.dki_train_epochs, "train_seconds": train_s,
