@@ -127,8 +127,8 @@ __global__ void __launch_bounds__(256) k_ivim_canonicalize(T* p, long long N) {
 }  // namespace
 
 extern "C" int pca_ivim_canonicalize(void* params, int64_t N, int32_t is_double, void* stream) {
+  if (N == 0) return PCA_OK;   // no voxels: empty arrays may have no address
   if (!params || N < 0) return pca_fail(PCA_ERR_ARG, "pca_ivim_canonicalize: bad argument");
-  if (N == 0) return PCA_OK;
   const dim3 grid((unsigned)((N + 255) / 256)), block(256);
   if (is_double)
     hipLaunchKernelGGL(k_ivim_canonicalize<double>, grid, block, 0, (hipStream_t)stream,
@@ -142,9 +142,10 @@ extern "C" int pca_ivim_canonicalize(void* params, int64_t N, int32_t is_double,
 
 extern "C" int pca_bpolyfit(const void* x, int32_t x_per_voxel, const void* y, int32_t m,
                             int32_t deg, void* out, int64_t N, int32_t is_double, void* stream) {
-  if (!x || !y || !out || m <= 0 || deg < 0 || deg > kMaxDeg || N < 0)
+  if (m <= 0 || deg < 0 || deg > kMaxDeg || N < 0)
     return pca_fail(PCA_ERR_ARG, "pca_bpolyfit: bad argument (deg must be 0..7)");
-  if (N == 0) return PCA_OK;
+  if (N == 0) return PCA_OK;   // no voxels: empty arrays may have no address
+  if (!x || !y || !out) return pca_fail(PCA_ERR_ARG, "pca_bpolyfit: NULL argument");
   const dim3 grid((unsigned)((N + 255) / 256)), block(256);
   if (is_double)
     hipLaunchKernelGGL(k_bpolyfit<double>, grid, block, 0, (hipStream_t)stream,
@@ -162,13 +163,14 @@ extern "C" int pca_bpolyfit(const void* x, int32_t x_per_voxel, const void* y, i
 extern "C" int pca_loglinear_init(const double* P, int32_t K, int32_t M, const void* y,
                                   int32_t kind, int32_t n, const void* lb, const void* ub,
                                   void* out, int64_t N, int32_t is_double, void* stream) {
-  if (!P || !y || !lb || !ub || !out || K <= 0 || K > kMaxK || M <= 0 || n <= 0 || n > kMaxK ||
-      kind < 0 || kind > 3 || N < 0)
+  if (!P || K <= 0 || K > kMaxK || M <= 0 || n <= 0 || n > kMaxK || kind < 0 || kind > 3 ||
+      N < 0)
     return pca_fail(PCA_ERR_ARG, "pca_loglinear_init: bad argument");
   if ((kind == 1 && (K != 2 || n != 2)) || (kind == 2 && (K != 2 || n != 3)) ||
       (kind == 3 && (K != 4 || n != 4)) || (kind == 0 && K != n))
     return pca_fail(PCA_ERR_ARG, "pca_loglinear_init: K/n do not match the init kind");
-  if (N == 0) return PCA_OK;
+  if (N == 0) return PCA_OK;   // no voxels: empty arrays may have no address
+  if (!y || !lb || !ub || !out) return pca_fail(PCA_ERR_ARG, "pca_loglinear_init: NULL argument");
   const dim3 grid((unsigned)((N + 255) / 256)), block(256);
   hipStream_t st = (hipStream_t)stream;
 #define PCA_INIT(TT, KK)                                                                      \

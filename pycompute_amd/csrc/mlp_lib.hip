@@ -633,8 +633,11 @@ extern "C" int pca_mlp_to_f16(const float* x, int16_t* y, int64_t n, void* strea
 }
 
 extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
-  if (!args || !args->data || !args->w1 || !args->w2 || !args->w3 || !args->params ||
-      !args->lb || !args->ub || !args->mu || !args->sd || !args->b1 || !args->b2 || !args->b3)
+  if (!args || !args->w1 || !args->w2 || !args->w3 || !args->mu || !args->sd || !args->b1 ||
+      !args->b2 || !args->b3)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: NULL argument");
+  // per-voxel arrays: empty ones (N = 0) may have no address
+  if (args->N > 0 && (!args->data || !args->params || !args->lb || !args->ub))
     return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: NULL argument");
   const int H = args->hidden, IN = args->in, OUT = args->out;
   if (IN < 1 || IN > 64 || OUT < 1 || OUT > 32 || (H != 128 && H != 256) || args->N < 0 ||

@@ -155,8 +155,9 @@ int pca_module_kernel(pca_module module, const char* name, pca_kernel* out) {
 
 int pca_lm_launch(pca_kernel kernel, const pca_lm_args* args, const void* uconsts,
                   size_t uconsts_bytes, uint32_t block, void* stream) {
-  if (!args || !args->data || !args->lb || !args->ub || !args->params || !args->lam ||
-      !args->step_type || !args->f)
+  // per-voxel arrays: empty ones (N = 0) may have no address
+  if (!args || (args->N > 0 && (!args->data || !args->lb || !args->ub || !args->params ||
+                                !args->lam || !args->step_type || !args->f)))
     return fail(PCA_ERR_ARG, "pca_lm_launch: a required array is NULL");
   if (args->iters < 0) return fail(PCA_ERR_ARG, "pca_lm_launch: iters < 0");
   UConsts uc;
@@ -168,7 +169,7 @@ int pca_lm_launch(pca_kernel kernel, const pca_lm_args* args, const void* uconst
 
 int pca_ssr_launch(pca_kernel kernel, const pca_ssr_args* args, const void* uconsts,
                    size_t uconsts_bytes, uint32_t block, void* stream) {
-  if (!args || !args->params || !args->data || !args->f)
+  if (!args || (args->N > 0 && (!args->params || !args->data || !args->f)))
     return fail(PCA_ERR_ARG, "pca_ssr_launch: a required array is NULL");
   UConsts uc;
   int rc = fill_uconsts(uc, uconsts, uconsts_bytes);
@@ -179,7 +180,8 @@ int pca_ssr_launch(pca_kernel kernel, const pca_ssr_args* args, const void* ucon
 
 int pca_eval_launch(pca_kernel kernel, const pca_eval_args* args, const void* uconsts,
                     size_t uconsts_bytes, uint32_t block, void* stream) {
-  if (!args || !args->params) return fail(PCA_ERR_ARG, "pca_eval_launch: params is NULL");
+  if (!args || (args->N > 0 && !args->params))
+    return fail(PCA_ERR_ARG, "pca_eval_launch: params is NULL");
   UConsts uc;
   int rc = fill_uconsts(uc, uconsts, uconsts_bytes);
   if (rc) return rc;

@@ -123,6 +123,8 @@ def split_consts(consts_t, nconst, M, N, uniform=None):
         return c.reshape(-1).detach().cpu().tolist(), None
     if c.dim() != 2 or c.shape[0] != ncm or c.shape[1] not in (1, N):
         raise ValueError(f"consts has shape {tuple(consts_t.shape)}, expected ({ncm}, {N})")
+    if N == 0 and c.shape[1] == 0:             # no voxels: nothing to tell apart
+        return None, c.contiguous()
     if c.shape[1] == 1 or (N > 0 and c.stride(1) == 0):
         return c[:, 0].detach().cpu().tolist(), None
     if uniform is None:
@@ -334,9 +336,9 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         """Run ``iters`` LM iterations on every voxel (one launch on the current
         stream; asynchronous like the reference). Voxels with step_type 0 are
         converged and skipped."""
-        if self.Nprobs <= 0 or not hasattr(self, "data_t"):
+        if not hasattr(self, "data_t"):
             raise RuntimeError("setup() must be called before run()")
-        if iters <= 0:
+        if iters <= 0 or self.Nprobs == 0:       # nothing to fit
             return None
         if not self.expr_true:
             return self._plugin.run(self, iters, tol, mu, eta, acc, dec)

@@ -53,6 +53,7 @@ def test_hiprtc_compile_ok_and_error():
 def test_argument_errors_without_gpu():
     L = _lib.lib()
     a = _lib.LMArgs()
+    a.N = 10                                    # per-voxel arrays required for N > 0
     rc = L.pca_lm_launch(None, ctypes.byref(a), None, 0, 256, None)
     assert rc == -1 and b"NULL" in L.pca_last_error()
     L.pca_bpolyfit.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
@@ -87,3 +88,28 @@ def test_mlp_infer_args_layout_and_validation():
     a.w1 = None
     assert L.pca_mlp_infer(ctypes.byref(a), None) == -1
     assert b"NULL" in L.pca_last_error()
+
+
+def test_zero_voxel_calls_accept_empty_arrays():
+    """With no voxels the per-voxel arrays of an empty tensor have no address;
+    the entry points return PCA_OK without launching (no GPU needed)."""
+    from pycompute_amd import mlp
+    L = _lib.lib()
+    P = (ctypes.c_double * 8)()
+    L.pca_loglinear_init.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]
+    assert L.pca_loglinear_init(ctypes.addressof(P), 2, 4, None, 1, 2, None, None, None, 0, 0,
+                                None) == 0
+    assert L.pca_loglinear_init(ctypes.addressof(P), 2, 4, None, 1, 2, None, None, None, 5, 0,
+                                None) == -1
+    L.pca_ivim_canonicalize.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                        ctypes.c_void_p]
+    assert L.pca_ivim_canonicalize(None, 0, 0, None) == 0
+    Lm = mlp._bind()
+    a = mlp.InferArgs(None, 1, 1, 1, 1, 1, 1, 1, 1, None, None, None, 0, 16, 4, 128, 0, 0, -1,
+                      12, 0, None, 0, 0)
+    assert Lm.pca_mlp_infer(ctypes.byref(a), None) == 0
+    a.N = 10
+    assert Lm.pca_mlp_infer(ctypes.byref(a), None) == -1
