@@ -11,6 +11,9 @@ of the caller's own problem:
   * fast_exp / fast_div: v_exp_f32 and the approximate FP32 divide (only when
                    the caller allows non-reference rounding)
   * reassoc:       FMA-folded Hessian update and reciprocal-based solve (idem)
+  * factored:      parameter-only factors of the Jacobian and second
+                   derivatives applied once per iteration (with reassoc)
+  * sched_group:   register fence every k measurements (bounds registers)
   * waves_per_eu:  occupancy hint (mixed-model batches, batch_candidates())
   * tile:          wide-model kernel register/LDS split (wide_candidates())
 
@@ -30,7 +33,7 @@ import torch
 from . import program
 
 DEFAULT = {"block": 256, "chunk": 16, "unroll": 0, "fast_exp": False, "fast_div": False,
-           "reassoc": False, "waves_per_eu": 0}
+           "reassoc": False, "waves_per_eu": 0, "factored": False, "sched_group": 0}
 
 
 def _cache_path() -> str:
@@ -61,17 +64,21 @@ def key(expr: str, M: int, dtype: str, allow_fast: bool, arch: str) -> str:
 def candidates(nparam: int, allow_fast: bool):
     """The single-model knob sets. The FMA-folded build runs with an occupancy
     hint of 2 waves per SIMD too: it needs a few registers more than 256,
-    which would otherwise halve occupancy."""
+    which would otherwise halve occupancy. The factored build runs unfenced
+    and with a fence every 4 measurements (about 120 registers, 4 waves per
+    SIMD)."""
     out = []
-    fast = [(False, False, False, 0)]
+    fast = [(False, False, False, 0, False, 0)]
     if allow_fast:
-        fast += [(True, True, False, 0), (True, False, False, 0), (True, True, True, 0),
-                 (True, True, True, 2)]
-    for fe, fd, ra, wpe in fast:
+        fast += [(True, True, False, 0, False, 0), (True, False, False, 0, False, 0),
+                 (True, True, True, 0, False, 0), (True, True, True, 2, False, 0),
+                 (True, True, True, 2, True, 0), (True, True, True, 0, True, 4)]
+    for fe, fd, ra, wpe, fac, sg in fast:
         for chunk in (8, 16, 24):
             for block in (256, 128):
                 out.append({"block": block, "chunk": chunk, "unroll": 0, "fast_exp": fe,
-                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe})
+                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe,
+                            "factored": fac, "sched_group": sg})
     return out
 
 
@@ -80,13 +87,15 @@ def batch_candidates(allow_fast: bool = True):
     register allocation is the largest model's; the occupancy hint (waves per
     SIMD) trades that against spills."""
     out = []
-    fast = [(False, False, False)] + ([(True, True, False), (True, True, True)]
-                                      if allow_fast else [])
-    for fe, fd, ra in fast:
+    fast = [(False, False, False, False)] + (
+        [(True, True, False, False), (True, True, True, False), (True, True, True, True)]
+        if allow_fast else [])
+    for fe, fd, ra, fac in fast:
         for chunk in (8, 16):
             for wpe in (0, 2):
                 out.append({"block": 256, "chunk": chunk, "unroll": 0, "fast_exp": fe,
-                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe})
+                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe,
+                            "factored": fac, "sched_group": 0})
     return out
 
 

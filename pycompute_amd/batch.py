@@ -73,7 +73,8 @@ class BatchedLM:
     def __init__(self, models, dtype=torch.float32, block: int = lm.DEFAULT_BLOCK,
                  chunk: int = 8, fast_exp: bool = False, fast_div: bool = False,
                  strict: bool = False, unroll: int = 0, waves_per_eu: int = 0,
-                 reassoc: bool = False, compaction: bool = True, **_):
+                 reassoc: bool = False, sched_group: int = 0, factored: bool = False,
+                 compaction: bool = True, **_):
         if not 1 <= len(models) <= 4:
             raise ValueError("a batch holds 1 to 4 models")
         self.dtype_name = lm.dtype_name(dtype)
@@ -86,7 +87,8 @@ class BatchedLM:
                                    self.dtype_name, self.block, False, bool(fast_exp),
                                    bool(strict), unroll=int(unroll),
                                    waves_per_eu=int(waves_per_eu), fast_div=bool(fast_div),
-                                   reassoc=bool(reassoc)))
+                                   reassoc=bool(reassoc), sched_group=int(sched_group),
+                                   factored=bool(factored)))
         self.members = [Member(s) for s in specs]
         self.program = lm.get_program(specs[0], tuple(specs[1:]))
         self.funcid = self.program.funcid

@@ -37,6 +37,28 @@
 #else
 #define PCA_KATTR
 #endif
+// Register fence every PCA_SCHED_GROUP measurements of the unrolled loops:
+// the running sums pass through an empty asm statement, then the parameters
+// do. The next group's arithmetic reads the fenced parameters, so it cannot be
+// scheduled ahead of this group's sums; the number of measurements in flight,
+// and the registers they hold, stay bounded.
+#ifndef PCA_SCHED_GROUP
+#define PCA_SCHED_GROUP 0
+#endif
+// Factored accumulation (the model struct's prep/accf/finish, rprep/residf):
+// parameter-only factors of the Jacobian and second derivatives are applied
+// once per iteration instead of once per measurement.
+#ifndef PCA_FACTORED
+#define PCA_FACTORED 0
+#endif
+namespace pca {
+__device__ __forceinline__ void reg_fence(real& x) { asm volatile("" : "+v"(x)); }
+template <int K>
+__device__ __forceinline__ void reg_fence(real (&x)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) reg_fence(x[k]);
+}
+}  // namespace pca
 #ifndef PCA_NMODELS
 #define PCA_NMODELS 1
 typedef Model Model0;
@@ -63,21 +85,62 @@ struct Voxel {
 template <class Mdl, int M, int UNROLL, class CF>
 __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, const CF& cf) {
   constexpr int NP = Mdl::NP, NC = Mdl::NC, NH = Mdl::NH;
+  constexpr bool factored = PCA_FACTORED && Mdl::FACTORED;
   // fghhl (lsqnonlin.py:206-346)
   V.f = 0;
+  if constexpr (factored) {
+    // the same sums with the parameter-only factors applied after the loop
+    typename Mdl::Pre P;
+    Mdl::prep(V.p, V.lam, P);
+    real s[Mdl::NS], q[NH], dmp[NP];
 #pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    V.g[j] = 0;
-    V.hld[j] = 0;
-  }
+    for (int k = 0; k < Mdl::NS; ++k) s[k] = 0;
 #pragma unroll
-  for (int l = 0; l < NH; ++l) V.h[l] = 0;
+    for (int l = 0; l < NH; ++l) q[l] = 0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) dmp[j] = 0;
 #pragma unroll UNROLL
-  for (int i = 0; i < M; ++i) {
-    real c[NC];
+    for (int i = 0; i < M; ++i) {
+      real c[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
-    Mdl::accum(V.p, c, V.y[i], V.lam, V.f, V.g, V.h, V.hld);
+      for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+      Mdl::accf(P, c, V.y[i], V.f, s, q, dmp);
+      if constexpr (PCA_SCHED_GROUP > 0) {
+        if (i % PCA_SCHED_GROUP == PCA_SCHED_GROUP - 1 && i + 1 < M) {
+          reg_fence(V.f);
+          reg_fence(s);
+          reg_fence(q);
+          reg_fence(dmp);
+          reg_fence(P.a);
+          reg_fence(P.L);
+        }
+      }
+    }
+    Mdl::finish(P, V.p, s, q, dmp, V.g, V.h, V.hld);
+  } else {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      V.g[j] = 0;
+      V.hld[j] = 0;
+    }
+#pragma unroll
+    for (int l = 0; l < NH; ++l) V.h[l] = 0;
+#pragma unroll UNROLL
+    for (int i = 0; i < M; ++i) {
+      real c[NC];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+      Mdl::accum(V.p, c, V.y[i], V.lam, V.f, V.g, V.h, V.hld);
+      if constexpr (PCA_SCHED_GROUP > 0) {
+        if (i % PCA_SCHED_GROUP == PCA_SCHED_GROUP - 1 && i + 1 < M) {
+          reg_fence(V.f);
+          reg_fence(V.g);
+          reg_fence(V.h);
+          reg_fence(V.hld);
+          reg_fence(V.p);
+        }
+      }
+    }
   }
   if constexpr (Mdl::HLD_DAMP_ONLY) {   // the fast build accumulates damping only
 #pragma unroll
@@ -118,13 +181,38 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
 #pragma unroll
   for (int j = 0; j < NP; ++j) ptp[j] = V.p[j] - V.s[j];
   real ftp = 0;
+  if constexpr (factored) {
+    typename Mdl::RPre R;
+    Mdl::rprep(ptp, R);
 #pragma unroll UNROLL
-  for (int i = 0; i < M; ++i) {
-    real c[NC];
+    for (int i = 0; i < M; ++i) {
+      real c[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
-    const real r = Mdl::resid(ptp, c, V.y[i]);
-    ftp += r * r;
+      for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+      const real r = Mdl::residf(R, c, V.y[i]);
+      ftp += r * r;
+      if constexpr (PCA_SCHED_GROUP > 0) {
+        if (i % PCA_SCHED_GROUP == PCA_SCHED_GROUP - 1 && i + 1 < M) {
+          reg_fence(ftp);
+          reg_fence(R.a);
+        }
+      }
+    }
+  } else {
+#pragma unroll UNROLL
+    for (int i = 0; i < M; ++i) {
+      real c[NC];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+      const real r = Mdl::resid(ptp, c, V.y[i]);
+      ftp += r * r;
+      if constexpr (PCA_SCHED_GROUP > 0) {
+        if (i % PCA_SCHED_GROUP == PCA_SCHED_GROUP - 1 && i + 1 < M) {
+          reg_fence(ftp);
+          reg_fence(ptp);
+        }
+      }
+    }
   }
 
   // gain ratio and step type (lsqnonlin.py:432-498)

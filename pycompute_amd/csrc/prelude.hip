@@ -57,6 +57,14 @@ __device__ __forceinline__ double pca_pow(double x, double y) { return pow(x, y)
 #define PCA_POW(x, y) pca_pow(x, y)
 #endif
 
+// exp(k*x) for a loop-invariant k (the factored accumulation): the fast-exp
+// float build folds log2(e) into k, so the loop pays one multiply.
+#if defined(PCA_FAST_EXP) && !defined(PCA_REAL_DOUBLE)
+#define PCA_EXPK(k, x) __builtin_amdgcn_exp2f(((k) * 1.44269504088896341f) * (x))
+#else
+#define PCA_EXPK(k, x) PCA_EXP((k) * (x))
+#endif
+
 __device__ __forceinline__ float pca_fmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double pca_fmax(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float pca_abs(float a) { return fabsf(a); }

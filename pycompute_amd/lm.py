@@ -54,6 +54,13 @@ class LMSpec:
     # reassociated FMA form of the Hessian update and a reciprocal-based
     # solve (fast builds; results equal the reference's up to rounding)
     reassoc: bool = False
+    # register fence every sched_group measurements of the unrolled loops
+    # (0 = none): bounds the measurements the compiler keeps in flight, so
+    # fewer registers and more waves per SIMD
+    sched_group: int = 0
+    # factored accumulation (with reassoc): parameter-only factors applied
+    # once per iteration instead of once per measurement
+    factored: bool = False
 
     @property
     def nparam(self):
@@ -98,6 +105,8 @@ class LMProgram:
         self.source = build_source(spec, self.dag, list(zip(self.members, self.member_dags)))
         if spec.strict and (spec.fast_exp or spec.fast_div or spec.reassoc):
             raise ValueError("the strict build excludes fast_exp, fast_div and reassoc")
+        if spec.factored and not spec.reassoc:
+            raise ValueError("the factored accumulation needs reassoc")
         options = list(BASE_OPTIONS)
         if spec.strict:
             options.append("-ffp-contract=off")
@@ -153,11 +162,16 @@ def build_source(spec: LMSpec, dag, members=()) -> str:
         head.append(f"#define PCA_WAVES_PER_EU {spec.waves_per_eu}")
     if spec.reassoc:
         head.append("#define PCA_FAST_SOLVE 1")
+    if spec.factored:
+        head.append("#define PCA_FACTORED 1")
+    if spec.sched_group > 0:
+        head.append(f"#define PCA_SCHED_GROUP {spec.sched_group}")
     structs, typedefs = {}, []
     for i, (s, d) in enumerate(models):
         sname = "PcaModel_" + hipgen.expr_hash(s.expr, 12)
         if sname not in structs:
-            structs[sname] = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc)
+            structs[sname] = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc,
+                                                factored=spec.factored)
         if members:
             typedefs.append(f"typedef {sname} Model{i};")
         if i == 0:

@@ -165,7 +165,7 @@ class SecondOrderRandomSearch(_SearchBase):
                  strict: bool = False, batch: int | None = None, **kwargs):
         lm_kw = {k: kwargs.pop(k) for k in list(kwargs)
                  if k in ("compaction", "chunk", "fast_exp", "fast_div", "reassoc", "unroll",
-                          "waves_per_eu", "wide")}
+                          "waves_per_eu", "sched_group", "factored", "wide")}
         super().__init__(*args, seed=seed, block=block, strict=strict, **kwargs)
         if batch is not None and int(batch) < 1:
             raise ValueError("batch must be at least 1")

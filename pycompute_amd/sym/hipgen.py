@@ -138,30 +138,32 @@ class _Printer:
         for n in E.topo(outputs):
             if n.op in ("num", "sym") or n.id in self.names:
                 continue
-            a = [self.ref(x) for x in n.args]
-            if n.op == "add":
-                txt = f"{a[0]} + {a[1]}"
-            elif n.op == "neg":
-                txt = f"-{a[0]}"
-            elif n.op == "mul":
-                txt = f"{a[0]}*{a[1]}"
-            elif n.op == "div":
-                txt = f"{a[0]}/{a[1]}"
-            elif n.op == "pow":
-                e = E.int_power(n)
-                if e is not None:
-                    txt = self._int_pow(a[0], abs(e))
-                    if e < 0:
-                        txt = f"{_lit(1.0, self.dtype)}/({txt})"
-                elif n.args[1].op == "num" and n.args[1].value == 0.5:
-                    txt = f"{self.fn['sqrt']}({a[0]})"
-                else:
-                    txt = f"{self.pow}({a[0]}, {a[1]})"
-            elif n.op == "fn":
-                txt = f"{self.fn[n.name]}({a[0]})"
-            else:  # pragma: no cover
-                raise ValueError(n.op)
-            self.names[n.id] = self._tmp(txt)
+            self.names[n.id] = self._tmp(self._format(n, [self.ref(x) for x in n.args]))
+
+    def _format(self, n: E.Node, a: list) -> str:
+        if n.op == "add":
+            txt = f"{a[0]} + {a[1]}"
+        elif n.op == "neg":
+            txt = f"-{a[0]}"
+        elif n.op == "mul":
+            txt = f"{a[0]}*{a[1]}"
+        elif n.op == "div":
+            txt = f"{a[0]}/{a[1]}"
+        elif n.op == "pow":
+            e = E.int_power(n)
+            if e is not None:
+                txt = self._int_pow(a[0], abs(e))
+                if e < 0:
+                    txt = f"{_lit(1.0, self.dtype)}/({txt})"
+            elif n.args[1].op == "num" and n.args[1].value == 0.5:
+                txt = f"{self.fn['sqrt']}({a[0]})"
+            else:
+                txt = f"{self.pow}({a[0]}, {a[1]})"
+        elif n.op == "fn":
+            txt = f"{self.fn[n.name]}({a[0]})"
+        else:  # pragma: no cover
+            raise ValueError(n.op)
+        return txt
 
     def _int_pow(self, x: str, k: int) -> str:
         # exponentiation by squaring on SSA temporaries
@@ -176,8 +178,231 @@ class _Printer:
         return result
 
 
+class _Factorer:
+    """Splits a product into its numeric coefficient and its parameter-only,
+    const-only and mixed factors: n = coef * prod(P) * prod(K) * prod(U)."""
+
+    def __init__(self, dag: ModelDAG):
+        self.g = dag.graph
+        self.pars, self.consts = frozenset(dag.pars), frozenset(dag.consts)
+        self._fs: dict = {}
+
+    def fs(self, n: E.Node) -> frozenset:
+        r = self._fs.get(n.id)
+        if r is None:
+            for x in E.topo([n]):
+                if x.id in self._fs:
+                    continue
+                if x.op == "sym":
+                    self._fs[x.id] = frozenset([x.name])
+                else:
+                    self._fs[x.id] = frozenset().union(*[self._fs[a.id] for a in x.args])
+            r = self._fs[n.id]
+        return r
+
+    def param_only(self, n: E.Node) -> bool:
+        return n.op != "num" and self.fs(n) <= self.pars
+
+    def split(self, n: E.Node):
+        coef, P, K, U = 1.0, [], [], []
+        stack = [n]
+        while stack:
+            x = stack.pop()
+            if x.op == "num":
+                coef *= x.value
+            elif x.op == "neg":
+                coef = -coef
+                stack.append(x.args[0])
+            elif x.op == "mul":
+                stack.extend(x.args)
+            else:
+                f = self.fs(x)
+                (P if f <= self.pars else K if f <= self.consts else U).append(x)
+        return coef, P, K, U
+
+    def prod(self, nodes) -> E.Node:
+        out = self.g.num(1)
+        for x in sorted(nodes, key=lambda z: z.id):
+            out = self.g.mul(out, x)
+        return out
+
+    def factor(self, n: E.Node):
+        """(A, u) with n = A * u, A parameter-only (or a number) and u free
+        of parameter-only factors."""
+        coef, P, K, U = self.split(n)
+        return self.g.mul(self.g.num(coef), self.prod(P)), self.prod(K + U)
+
+
+class _MeasPrinter(_Printer):
+    """Per-measurement code of the factored accumulation: parameter-only
+    subexpressions are read from the per-iteration table ``{pre}.a[i]``
+    (collected in ``pre_nodes``), and exp(k*x) with a const-only factor k is
+    printed as PCA_EXPK(k, x) so that the scaled k is loop-invariant."""
+
+    def __init__(self, dag, dtype, fac: _Factorer, pre: str):
+        super().__init__(dag, dtype)
+        self.fac, self.pre_name = fac, pre
+        self.pre: dict = {}
+        self.pre_nodes: list = []
+        self._expk: dict = {}
+
+    def ref(self, n: E.Node) -> str:
+        if self.fac.param_only(n):
+            i = self.pre.get(n.id)
+            if i is None:
+                i = self.pre[n.id] = len(self.pre_nodes)
+                self.pre_nodes.append(n)
+            return f"{self.pre_name}.a[{i}]"
+        return super().ref(n)
+
+    def _children(self, n: E.Node):
+        if n.op in ("num", "sym") or self.fac.param_only(n):
+            return ()
+        if n.op == "fn" and n.name == "exp":
+            coef, P, K, U = self.fac.split(n.args[0])
+            if K:
+                g = self.fac.g
+                k = g.mul(g.num(coef), self.fac.prod(K))
+                x = g.mul(self.fac.prod(P), self.fac.prod(U))
+                self._expk[n.id] = (k, x)
+                return (k, x)
+        return n.args
+
+    def emit(self, outputs):
+        seen = set()
+        for o in outputs:
+            stack = [(o, False)]
+            while stack:
+                node, done = stack.pop()
+                if done:
+                    if node.op in ("num", "sym") or self.fac.param_only(node):
+                        continue
+                    if node.id in self._expk:
+                        k, x = self._expk[node.id]
+                        txt = f"PCA_EXPK({self.ref(k)}, {self.ref(x)})"
+                    else:
+                        txt = self._format(node, [self.ref(a) for a in node.args])
+                    self.names[node.id] = self._tmp(txt)
+                    continue
+                if node.id in seen:
+                    continue
+                seen.add(node.id)
+                stack.append((node, True))
+                for c in reversed(self._children(node)):
+                    if c.id not in seen:
+                        stack.append((c, False))
+
+
+def _emit_factored(dag: ModelDAG, dtype: str) -> list:
+    """Members of the factored accumulation (fast builds).
+
+    Every Jacobian entry and second derivative X is split as X = A(p) * u,
+    A the product of its parameter-only factors (and sign), u the rest. The
+    measurement loop then accumulates only u-products, and the per-iteration
+    factors are applied once, after the loop:
+        g_j  = A_j  * sum u_j r
+        h_jk = A_j A_k * sum u_j u_k + A''_jk * sum u''_jk r
+        hld_j = sum max(L_j u_j^2, 1e-4),  L_j = lam A_j^2
+    Sums over the same u are shared (a second derivative often has the u of a
+    Jacobian entry). The model m = A_m u_m gives r = fma(A_m, u_m, -y). Equal
+    to accum() up to rounding."""
+    n, nh = dag.nparam, dag.nhes
+    fac = _Factorer(dag)
+    g = dag.graph
+    floor = _lit(1e-4, dtype)
+    nz = dag.hes_nonzero()
+    mA, mu = fac.factor(dag.model)
+    jf = [fac.factor(J) for J in dag.jac]
+    hf = [fac.factor(h) if z else None for h, z in zip(dag.hes, nz)]
+
+    # one accumulator per distinct u multiplied by r
+    sidx: dict = {}
+    for A, u in jf + [x for x in hf if x is not None]:
+        sidx.setdefault(u.id, (len(sidx), u))
+
+    def resid(pr):
+        if mA.op == "num" and mA.value == 1.0:
+            return f"{pr.ref(mu)} - y"
+        return f"pca_fma({pr.ref(mA)}, {pr.ref(mu)}, -y)"
+
+    # accf: one measurement
+    pm = _MeasPrinter(dag, dtype, fac, "P")
+    pm.emit([mu] + [u for _, u in jf] + [x[1] for x in hf if x is not None])
+    acc = list(pm.lines)
+    acc.append(f"const real r = {resid(pm)};")
+    acc.append("f = pca_fma(r, r, f);")
+    for i, u in sidx.values():
+        acc.append(f"s[{i}] = pca_fma({pm.ref(u)}, r, s[{i}]);")
+    l = 0
+    for j in range(n):
+        uj = pm.ref(jf[j][1])
+        for k in range(j + 1):
+            if j == k:
+                acc.append(f"{{ const real uu = {uj}*{uj}; q[{l}] += uu; "
+                           f"dmp[{j}] += pca_fmax(P.L[{j}]*uu, {floor}); }}")
+            else:
+                acc.append(f"q[{l}] = pca_fma({uj}, {pm.ref(jf[k][1])}, q[{l}]);")
+            l += 1
+
+    # prep: the per-iteration table and lam A_j^2
+    pp = _Printer(dag, dtype)
+    pp.emit(pm.pre_nodes + [A for A, _ in jf])
+    prep = list(pp.lines)
+    prep += [f"P.a[{i}] = {pp.ref(x)};" for i, x in enumerate(pm.pre_nodes)]
+    prep += [f"{{ const real a = {pp.ref(A)}; P.L[{j}] = lam*(a*a); }}"
+             for j, (A, _) in enumerate(jf)]
+
+    # finish: apply the factors
+    pf = _Printer(dag, dtype)
+    prods, l = [], 0
+    for j in range(n):
+        for k in range(j + 1):
+            prods.append(g.mul(jf[j][0], jf[k][0]))
+    pf.emit([A for A, _ in jf] + prods + [x[0] for x in hf if x is not None])
+    fin = list(pf.lines)
+    for j in range(n):
+        fin.append(f"g[{j}] = {pf.ref(jf[j][0])}*s[{sidx[jf[j][1].id][0]}];")
+        fin.append(f"hld[{j}] = dmp[{j}];")
+    for l in range(nh):
+        gn = f"{pf.ref(prods[l])}*q[{l}]"
+        if hf[l] is None:
+            fin.append(f"h[{l}] = {gn};")
+        else:
+            A2, u2 = hf[l]
+            fin.append(f"h[{l}] = pca_fma({pf.ref(A2)}, s[{sidx[u2.id][0]}], {gn});")
+
+    # trial residual: rprep once per trial point, residf per measurement
+    pr = _MeasPrinter(dag, dtype, fac, "R")
+    pr.emit([mu])
+    rf = list(pr.lines) + [f"return {resid(pr)};"]
+    prr = _Printer(dag, dtype)
+    prr.emit(pr.pre_nodes)
+    rp = list(prr.lines) + [f"R.a[{i}] = {prr.ref(x)};" for i, x in enumerate(pr.pre_nodes)]
+
+    ind = lambda lines: ["    " + x for x in lines]
+    out = ["  static constexpr bool FACTORED = true;",
+           f"  static constexpr int NS = {max(1, len(sidx))};",
+           f"  struct Pre {{ real a[{max(1, len(pm.pre_nodes))}]; real L[{n}]; }};",
+           f"  struct RPre {{ real a[{max(1, len(pr.pre_nodes))}]; }};",
+           "  __device__ static __forceinline__ void prep(const real* __restrict__ p, real lam, "
+           "Pre& P) {"] + ind(prep) + ["  }"]
+    out += ["  __device__ static __forceinline__ void accf(const Pre& P, "
+            "const real* __restrict__ c, real y, real& f, real* s, real* q, real* dmp) {"]
+    out += ind(acc) + ["  }"]
+    out += ["  __device__ static __forceinline__ void finish(const Pre& P, "
+            "const real* __restrict__ p, const real* s, const real* q, const real* dmp, "
+            "real* g, real* h, real* hld) {"]
+    out += ind(fin) + ["  }"]
+    out += ["  __device__ static __forceinline__ void rprep(const real* __restrict__ p, RPre& R) {"]
+    out += ind(rp) + ["  }"]
+    out += ["  __device__ static __forceinline__ real residf(const RPre& R, "
+            "const real* __restrict__ c, real y) {"]
+    out += ind(rf) + ["  }"]
+    return out
+
+
 def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
-               reassoc: bool = False) -> str:
+               reassoc: bool = False, factored: bool = False) -> str:
     """HIP source of the model struct (see module docstring).
 
     reassoc=False keeps the reference's association of the per-measurement
@@ -185,7 +410,11 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
     reassoc=True (fast builds) folds it into FMAs, h = fma(J_j, J_k,
     fma(d2f, r, h)), and accumulates only the damping terms into hld (the
     kernel adds the diagonal of h after the measurement loop,
-    HLD_DAMP_ONLY): fewer instructions, results equal up to rounding."""
+    HLD_DAMP_ONLY): fewer instructions, results equal up to rounding.
+    factored=True (with reassoc) adds the factored accumulation
+    (_emit_factored), FACTORED = true."""
+    if factored and not reassoc:
+        raise ValueError("the factored accumulation is a reassociated (fast) form")
     n, nh = dag.nparam, dag.nhes
     floor = _lit(1e-4, dtype)
     out = [f"struct {struct_name} {{",
@@ -251,6 +480,8 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
                "const real* __restrict__ c, real& m, real* J, real* d2) {")
     out += ["    " + s for s in body]
     out.append("  }")
+    out += _emit_factored(dag, dtype) if factored else \
+        ["  static constexpr bool FACTORED = false;"]
     out.append("};")
     return "\n".join(out) + "\n"
 

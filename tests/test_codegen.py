@@ -178,3 +178,93 @@ def test_hiprtc_compiles_for_gfx950(name, M, dtype, strict):
     prog = lm.get_program(lm.LMSpec(m.expr, m.pars, m.consts, M, dtype, strict=strict))
     code = prog.module.code
     assert code[:4] == b"\x7fELF" and b"pca_lm_fit" in code and b"gfx950" in code
+
+
+FACTOR_EXPRS = USER_EXPRS + [(m.expr, m.pars, m.consts) for k, m in sorted(MODELS.items())
+                             if len(m.pars) <= 8] + [
+    ("p0*c**2 + p1*c*exp(-p2*c) - p0*p1", ["p0", "p1", "p2"], ["c"]),
+    ("-(a*b)*exp(-(k*x)*(a*a))/(1 + x)", ["a", "b"], ["k", "x"]),
+]
+
+
+@pytest.mark.parametrize("expr,pars,consts", FACTOR_EXPRS)
+def test_factored_split_is_exact(expr, pars, consts):
+    """Every Jacobian entry, second derivative and the model split as A(p) u:
+    A has only parameters, u has no parameter-only factor, and A u equals the
+    unsplit expression at random points."""
+    dag = hipgen.res_jac_hes(expr, pars, consts)
+    fac = hipgen._Factorer(dag)
+    rng = np.random.default_rng(5)
+    P, C = _points(dag, rng)
+    env = dict(zip(dag.pars, P))
+    env.update(zip(dag.consts, C))
+    memo = {}
+    nz = dag.hes_nonzero()
+    for X in [dag.model] + dag.jac + [h for h, z in zip(dag.hes, nz) if z]:
+        A, u = fac.factor(X)
+        assert A.op == "num" or fac.fs(A) <= set(dag.pars)
+        coef, Pf, K, U = fac.split(u)
+        assert not Pf and coef == 1.0
+        x = np.broadcast_to(E.evaluate(X, env, memo), (64,))
+        au = np.broadcast_to(E.evaluate(A, env, memo) * E.evaluate(u, env, memo), (64,))
+        np.testing.assert_allclose(au, x, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("expr,pars,consts", FACTOR_EXPRS)
+def test_factored_sums_equal_fghhl(expr, pars, consts):
+    """The factored sums (g = A sum u r, h = A_j A_k sum u_j u_k + A'' sum u'' r,
+    hld = sum max(lam A^2 u^2, 1e-4) + h_jj) reproduce the fghhl loop of the
+    reference (lsqnonlin.py:237-256) in float64."""
+    dag = hipgen.res_jac_hes(expr, pars, consts)
+    fac = hipgen._Factorer(dag)
+    rng = np.random.default_rng(6)
+    M, lam = 12, 0.37
+    p = [rng.uniform(0.3, 1.5) for _ in dag.pars]
+    cs = [rng.uniform(0.2, 2.0, M) for _ in dag.consts]
+    y = rng.uniform(0.0, 2.0, M)
+    n, nz = dag.nparam, dag.hes_nonzero()
+    env = dict(zip(dag.pars, p))
+    env.update(zip(dag.consts, cs))
+    ev = lambda node: np.broadcast_to(np.asarray(E.evaluate(node, env, {}), float), (M,))
+    r = ev(dag.model) - y
+    J = [ev(j) for j in dag.jac]
+    d2 = [ev(h) for h in dag.hes]
+    g_ref = np.array([np.sum(J[j] * r) for j in range(n)])
+    h_ref, hld_ref, l = [], np.zeros(n), 0
+    for j in range(n):
+        for k in range(j + 1):
+            h_ref.append(np.sum(d2[l] * r + J[j] * J[k]))
+            if j == k:
+                hld_ref[j] = np.sum(d2[l] * r + J[j] * J[k] + np.maximum(lam * J[j] ** 2, 1e-4))
+            l += 1
+    jf = [fac.factor(x) for x in dag.jac]
+    hf = [fac.factor(x) if z else None for x, z in zip(dag.hes, nz)]
+    A = [float(np.ravel(ev(a))[0]) for a, _ in jf]
+    u = [ev(b) for _, b in jf]
+    g = np.array([A[j] * np.sum(u[j] * r) for j in range(n)])
+    h, hld, l = [], np.zeros(n), 0
+    for j in range(n):
+        for k in range(j + 1):
+            v = A[j] * A[k] * np.sum(u[j] * u[k])
+            if hf[l] is not None:
+                v += float(np.ravel(ev(hf[l][0]))[0]) * np.sum(ev(hf[l][1]) * r)
+            h.append(v)
+            if j == k:
+                hld[j] = np.sum(np.maximum(lam * A[j] ** 2 * u[j] ** 2, 1e-4)) + v
+            l += 1
+    np.testing.assert_allclose(g, g_ref, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(h, h_ref, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(hld, hld_ref, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("name,M", [("ivim", 16), ("t2", 32), ("se", 16)])
+def test_hiprtc_compiles_factored(name, M):
+    from pycompute_amd import lm
+    m = MODELS[name]
+    for sg in (0, 4):
+        prog = lm.get_program(lm.LMSpec(m.expr, m.pars, m.consts, M, fast_exp=True,
+                                        fast_div=True, reassoc=True, factored=True,
+                                        sched_group=sg))
+        assert b"pca_lm_chunk" in prog.module.code
+    with pytest.raises(ValueError):
+        lm.get_program(lm.LMSpec(m.expr, m.pars, m.consts, M, factored=True))

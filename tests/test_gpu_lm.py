@@ -94,7 +94,10 @@ def test_t2_and_se_parity(gpu):
 
 
 FAST_BUILDS = {"fast": dict(fast_exp=True, fast_div=True),
-               "fast-reassoc": dict(fast_exp=True, fast_div=True, reassoc=True)}
+               "fast-reassoc": dict(fast_exp=True, fast_div=True, reassoc=True),
+               "fast-factored": dict(fast_exp=True, fast_div=True, reassoc=True, factored=True),
+               "fast-factored-fenced": dict(fast_exp=True, fast_div=True, reassoc=True,
+                                            factored=True, sched_group=4)}
 
 
 @pytest.mark.parametrize("build", sorted(FAST_BUILDS))
@@ -108,6 +111,47 @@ def test_fast_builds_parity(gpu, build):
     for model, wm in (("t2", synth.t2(8192, seed=3)), ("se", synth.se(8192, seed=4))):
         prob, p0, g, r = _run_pair(model, wm, 100, **kw)
         _check(g, r, f"{model}-{build}")
+
+
+@pytest.mark.parametrize("model,n", [("ivim", 16384), ("t2", 8192), ("se", 8192)])
+@pytest.mark.parametrize("sched_group", [0, 4])
+def test_factored_gradient_and_hessian(gpu, model, n, sched_group):
+    """One iteration of the factored build leaves the same gradient, Hessian
+    and damped Hessian as the FMA-folded build, up to FP32 rounding. The
+    bounds are scale-aware, with D the damped diagonal (which holds lam
+    sum J_j^2 and so the size of the Gauss-Newton terms even where the
+    second-order terms cancel them in h_jj): |dh_jk| <= 1e-4 sqrt(D_j D_k),
+    |dg_j| <= 1e-4 sqrt(D_j f), |dD_j| <= 1e-4 D_j. A wrong index or sign in
+    the generated code breaks the bounds on nearly every voxel; FP32
+    cancellation leaves a few per thousand outside them (both forms replayed
+    in float32 on the CPU from the same start: 0.01%; the device's exp
+    rounds the two differently arranged arguments differently on top)."""
+    w = {"ivim": lambda: synth.ivim(n, synth.B_C2, seed=2, snr=(20.0, 100.0)),
+         "t2": lambda: synth.t2(n, seed=3), "se": lambda: synth.se(n, seed=4)}[model]()
+    prob = fit.make_problem(model, w["consts"], w["data"])
+    p0 = fit.default_init(prob)
+    out = {}
+    for tag, kw in (("ref", dict(reassoc=True)),
+                    ("fac", dict(reassoc=True, factored=True, sched_group=sched_group))):
+        s = fit.make_solver(prob, keep_workspace=True, fast_exp=True, fast_div=True, **kw)
+        s.setup(p0.clone(), prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+        s.run(1)
+        torch.cuda.synchronize()
+        out[tag] = {k: getattr(s, k).double().cpu().numpy() for k in ("g_t", "h_t", "hl_t")}
+        out[tag]["f"] = s.first_f.double().cpu().numpy().reshape(-1)
+    a, b = out["fac"], out["ref"]
+    npar = a["g_t"].shape[0]
+    diag = [j * (j + 1) // 2 + j for j in range(npar)]
+    hd = np.abs(b["hl_t"][diag])
+    ok = np.ones(a["g_t"].shape[1], bool)
+    for j in range(npar):
+        ok &= np.abs(a["g_t"][j] - b["g_t"][j]) <= 1e-4 * np.sqrt(hd[j] * b["f"]) + 1e-30
+        ok &= np.abs(a["hl_t"][diag[j]] - b["hl_t"][diag[j]]) <= 1e-4 * np.abs(b["hl_t"][diag[j]])
+        for k in range(j + 1):
+            l = j * (j + 1) // 2 + k
+            ok &= np.abs(a["h_t"][l] - b["h_t"][l]) <= 1e-4 * np.sqrt(hd[j] * hd[k]) + 1e-30
+    print(f"[{model} g={sched_group}] within bounds {ok.mean():.5f}", flush=True)
+    assert ok.mean() >= 0.998
 
 
 def test_ivim_float64_parity(gpu):
