@@ -23,10 +23,15 @@ def _solo(prob, p0, **kw):
     return fit.fit(prob, iters=100, params0=p0.clone(), **kw)
 
 
-@pytest.mark.parametrize("strict", [True, False])
-def test_batch_equals_separate_fits(gpu, strict):
+BUILDS = {"strict": dict(strict=True), "default": dict(strict=False),
+          "factored": dict(fast_exp=True, fast_div=True, reassoc=True, factored=True,
+                           waves_per_eu=2)}
+
+
+@pytest.mark.parametrize("build", sorted(BUILDS))
+def test_batch_equals_separate_fits(gpu, build):
     probs, p0 = _problems((3000, 5000, 1500))
-    kw = dict(strict=strict, chunk=8)
+    kw = dict(chunk=8, **BUILDS[build])
     b = BatchedLM([(MODELS[k], p.M) for k, p in zip(ORDER, probs)], torch.float32, **kw)
     pars = [q.clone() for q in p0]
     b.setup([(pars[i], p.consts_t, p.data_t, p.lb_t, p.ub_t) for i, p in enumerate(probs)])

@@ -194,14 +194,16 @@ def test_resume_equals_single_run(gpu):
     assert torch.equal(a.n_iter_t, b.n_iter_t)
 
 
-def test_per_voxel_consts_layout(gpu):
+@pytest.mark.parametrize("knobs", [{}, dict(fast_exp=True, fast_div=True, reassoc=True,
+                                            factored=True)], ids=["default", "factored"])
+def test_per_voxel_consts_layout(gpu, knobs):
     """The reference's replicated (1, M, N) consts layout gives the same fit."""
     w = synth.ivim_c1(2048, seed=1)
     prob = fit.make_problem("ivim", w["consts"], w["data"])
     p0 = fit.default_init(prob)
-    a = fit.fit(prob, iters=50, params0=p0.clone())
+    a = fit.fit(prob, iters=50, params0=p0.clone(), **knobs)
     cons = prob.consts_t.reshape(1, -1, 1).expand(1, prob.M, prob.N).contiguous()
-    b = fit.make_solver(prob)
+    b = fit.make_solver(prob, **knobs)
     b.setup(p0.clone(), cons, prob.data_t, prob.lb_t, prob.ub_t, uniform_consts=False)
     assert b.program.spec.consts_per_voxel
     b.run(50)
