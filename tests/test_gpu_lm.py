@@ -113,6 +113,33 @@ def test_fast_builds_parity(gpu, build):
         _check(g, r, f"{model}-{build}")
 
 
+def _factored_vs_folded(prob, p0, sched_group, tag):
+    """Fraction of voxels whose one-iteration gradient, Hessian and damped
+    Hessian agree between the factored and the FMA-folded fast builds."""
+    out = {}
+    for key, kw in (("ref", dict(reassoc=True)),
+                    ("fac", dict(reassoc=True, factored=True, sched_group=sched_group))):
+        s = fit.make_solver(prob, keep_workspace=True, fast_exp=True, fast_div=True, **kw)
+        s.setup(p0.clone(), prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+        s.run(1)
+        torch.cuda.synchronize()
+        out[key] = {k: getattr(s, k).double().cpu().numpy() for k in ("g_t", "h_t", "hl_t")}
+        out[key]["f"] = s.first_f.double().cpu().numpy().reshape(-1)
+    a, b = out["fac"], out["ref"]
+    npar = a["g_t"].shape[0]
+    diag = [j * (j + 1) // 2 + j for j in range(npar)]
+    hd = np.abs(b["hl_t"][diag])
+    ok = np.ones(a["g_t"].shape[1], bool)
+    for j in range(npar):
+        ok &= np.abs(a["g_t"][j] - b["g_t"][j]) <= 1e-4 * np.sqrt(hd[j] * b["f"]) + 1e-30
+        ok &= np.abs(a["hl_t"][diag[j]] - b["hl_t"][diag[j]]) <= 1e-4 * np.abs(b["hl_t"][diag[j]])
+        for k in range(j + 1):
+            l = j * (j + 1) // 2 + k
+            ok &= np.abs(a["h_t"][l] - b["h_t"][l]) <= 1e-4 * np.sqrt(hd[j] * hd[k]) + 1e-30
+    print(f"[{tag} g={sched_group}] within bounds {ok.mean():.5f}", flush=True)
+    return ok.mean()
+
+
 @pytest.mark.parametrize("model,n", [("ivim", 16384), ("t2", 8192), ("se", 8192)])
 @pytest.mark.parametrize("sched_group", [0, 4])
 def test_factored_gradient_and_hessian(gpu, model, n, sched_group):
@@ -129,29 +156,38 @@ def test_factored_gradient_and_hessian(gpu, model, n, sched_group):
     w = {"ivim": lambda: synth.ivim(n, synth.B_C2, seed=2, snr=(20.0, 100.0)),
          "t2": lambda: synth.t2(n, seed=3), "se": lambda: synth.se(n, seed=4)}[model]()
     prob = fit.make_problem(model, w["consts"], w["data"])
-    p0 = fit.default_init(prob)
-    out = {}
-    for tag, kw in (("ref", dict(reassoc=True)),
-                    ("fac", dict(reassoc=True, factored=True, sched_group=sched_group))):
-        s = fit.make_solver(prob, keep_workspace=True, fast_exp=True, fast_div=True, **kw)
-        s.setup(p0.clone(), prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
-        s.run(1)
-        torch.cuda.synchronize()
-        out[tag] = {k: getattr(s, k).double().cpu().numpy() for k in ("g_t", "h_t", "hl_t")}
-        out[tag]["f"] = s.first_f.double().cpu().numpy().reshape(-1)
-    a, b = out["fac"], out["ref"]
-    npar = a["g_t"].shape[0]
-    diag = [j * (j + 1) // 2 + j for j in range(npar)]
-    hd = np.abs(b["hl_t"][diag])
-    ok = np.ones(a["g_t"].shape[1], bool)
-    for j in range(npar):
-        ok &= np.abs(a["g_t"][j] - b["g_t"][j]) <= 1e-4 * np.sqrt(hd[j] * b["f"]) + 1e-30
-        ok &= np.abs(a["hl_t"][diag[j]] - b["hl_t"][diag[j]]) <= 1e-4 * np.abs(b["hl_t"][diag[j]])
-        for k in range(j + 1):
-            l = j * (j + 1) // 2 + k
-            ok &= np.abs(a["h_t"][l] - b["h_t"][l]) <= 1e-4 * np.sqrt(hd[j] * hd[k]) + 1e-30
-    print(f"[{model} g={sched_group}] within bounds {ok.mean():.5f}", flush=True)
-    assert ok.mean() >= 0.998
+    assert _factored_vs_folded(prob, fit.default_init(prob), sched_group, model) >= 0.995
+
+
+USER_MODELS = [
+    ("a*exp(-x/b) + c", ("a", "b", "c"), (1.0, 0.5, 0.1)),
+    ("a*sin(w*x + ph)**2 + log(1 + b*x)", ("a", "w", "ph", "b"), (1.0, 1.3, 0.4, 0.7)),
+    ("sqrt(a*x + b)/(1 + c*x**3) - a**2", ("a", "b", "c"), (0.8, 0.6, 0.3)),
+    ("p0*tanh(p1*x)^2 - p2/(x + p1)", ("p0", "p1", "p2"), (1.2, 0.9, 0.5)),
+]
+
+
+@pytest.mark.parametrize("expr,pars,truth", USER_MODELS, ids=[m[0] for m in USER_MODELS])
+def test_factored_user_expressions(gpu, expr, pars, truth):
+    """The same agreement for user expressions with sin, log, sqrt, tanh,
+    powers and quotients, whose factors the generator has to find inside
+    nested products (the model written through the reference's Expr form)."""
+    from pycompute_amd.models import ModelSpec
+    from pycompute_amd.sym import expr as E, hipgen
+    n, M = 8192, 24
+    spec = ModelSpec("user", expr, pars, ("x",), tuple(-10.0 for _ in pars),
+                     tuple(10.0 for _ in pars))
+    x = np.linspace(0.1, 2.5, M)
+    rng = np.random.default_rng(11)
+    P = np.array(truth)[:, None] * rng.uniform(0.8, 1.2, (len(pars), n))
+    dag = hipgen.res_jac_hes(expr, pars, ("x",))
+    env = {k: P[i][None, :] for i, k in enumerate(pars)}
+    env["x"] = x[:, None]
+    y = E.evaluate(dag.model, env) + rng.normal(0, 0.01, (M, n))
+    prob = fit.make_problem(spec, x[None, :], y)
+    p0 = torch.as_tensor(P * rng.uniform(0.9, 1.1, P.shape), device="cuda",
+                         dtype=torch.float32).contiguous()
+    assert _factored_vs_folded(prob, p0, 0, expr) >= 0.995
 
 
 def test_ivim_float64_parity(gpu):
