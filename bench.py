@@ -83,15 +83,18 @@ ITER_CAPS = (2, 3, 4, 5, 6, 8, 10, 12, 15, 20, 25, 30, 40, 50, 70, 100)
 def _traffic(knobs):
     """HBM bytes per fit of the chunked LM kernel, from the copy-calibrated
     FETCH_SIZE / WRITE_SIZE counter runs recorded in profiles/ for the same
-    knob set (None when the knobs differ or no record exists)."""
+    knob set (None when no record has these knobs)."""
     path = os.path.join(ROOT, "profiles", "pca_lm_chunk_ivim_1M.traffic.json")
     try:
         with open(path) as fh:
-            rec = json.load(fh)
+            recs = json.load(fh)
     except (OSError, ValueError):
         return None
     k = {key: v for key, v in knobs.items() if key != "keep_workspace"}
-    return rec["bytes_per_fit"] if rec.get("tuning") == k else None
+    for rec in (recs if isinstance(recs, list) else [recs]):
+        if rec.get("tuning") == k:
+            return rec["bytes_per_fit"]
+    return None
 
 
 def _time_replays(replay, args):
