@@ -162,6 +162,9 @@ int pca_lm_launch(pca_kernel kernel, const pca_lm_args* args, const void* uconst
 /* Chunked fit with convergence compaction over `nmodels` models (kernels
  * "pca_lm_chunk" and "pca_reset_counts" of one generated module): resets the
  * queue counters (device array of (n_chunks+1)*nmodels int32) with a kernel,
+ * or, when reset_kernel is NULL, relies on the caller having reset them on the
+ * same stream (row 0 = each model's N, the rest 0; the module's
+ * "pca_lm_reset_fit" does so together with the state of a single-model fit),
  * then launches one chunk per entry of chunk_iters (host array; their sum is
  * the iteration count) on a grid of `grid` blocks. Per-voxel results equal
  * those of pca_lm_launch with the same iteration count. Replaces the same

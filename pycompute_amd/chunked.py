@@ -85,23 +85,26 @@ class ChunkedRun:
         for i in range(len(self.jobs)):
             self.job_array[i].a.iters = int(iters)
 
-    def launch(self, schedule, stream=None):
+    def launch(self, schedule, stream=None, reset_counts=True):
+        """Launch the chunk sequence; reset_counts=False when the caller has
+        already reset the queue counters on this stream (pca_lm_reset_fit)."""
         if not schedule:
             return
         counts = self._counts_for(len(schedule))
         s = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
         k = (ctypes.c_int32 * len(schedule))(*schedule)
         L = _lib.lib()
-        _lib.check(L.pca_lm_chunked_launch(self.program.kernel("pca_lm_chunk"),
-                                           self.program.kernel("pca_reset_counts"),
+        reset = self.program.kernel("pca_reset_counts") if reset_counts else None
+        _lib.check(L.pca_lm_chunked_launch(self.program.kernel("pca_lm_chunk"), reset,
                                            self.job_array, len(self.jobs), counts.data_ptr(), k,
                                            len(schedule), self.grid, self.block, s),
                    "pca_lm_chunked_launch")
 
     # -- hipGraph -------------------------------------------------------------
-    def capture(self, schedule, prologue=None):
+    def capture(self, schedule, prologue=None, reset_counts=True):
         """Capture prologue() (e.g. the init kernel) and the chunk sequence into
-        a graph on a side stream; returns the graph handle."""
+        a graph on a side stream; returns the graph handle. reset_counts=False:
+        the prologue resets the queue counters itself."""
         self.destroy_graph()
         self._counts_for(len(schedule))
         self.program.kernel("pca_lm_chunk")
@@ -116,7 +119,7 @@ class ChunkedRun:
             try:
                 if prologue is not None:
                     prologue(h)
-                self.launch(schedule, stream=h)
+                self.launch(schedule, stream=h, reset_counts=reset_counts)
                 ok = True
             finally:
                 g = ctypes.c_void_p()

@@ -576,3 +576,20 @@ extern "C" __global__ void pca_lm_reset_state(pca_lm_args a) {
     if (a.n_iter) a.n_iter[v] = 0;
   }
 }
+
+// Both resets of a single-model fit in one launch (a captured fit then starts
+// with one kernel fewer): the state of pca_lm_reset_state, and the queue
+// counters of pca_reset_counts for a sequence of n / PCA_NMODELS - 1 chunks.
+extern "C" __global__ void pca_lm_reset_fit(pca_lm_args a, int* counts, int n,
+                                            pca_counts_init init) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long i = t0; i < n; i += stride)
+    counts[i] = i < PCA_NMODELS ? init.first[i] : 0;
+  for (long long v = t0; v < a.N; v += stride) {
+    a.lam[v] = (real)5;
+    a.step_type[v] = 1;
+    a.f[v] = (real)0;
+    if (a.n_iter) a.n_iter[v] = 0;
+  }
+}

@@ -203,7 +203,7 @@ int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
                           const pca_model_job* jobs, int32_t nmodels, int32_t* counts,
                           const int32_t* chunk_iters, int32_t n_chunks, uint32_t grid,
                           uint32_t block, void* stream) {
-  if (!chunk_kernel || !reset_kernel || !jobs || !counts || !chunk_iters)
+  if (!chunk_kernel || !jobs || !counts || !chunk_iters)
     return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: NULL argument");
   if (nmodels < 1 || nmodels > 4) return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: 1..4 models");
   if (n_chunks < 1 || grid == 0 || block == 0 || block > 1024 || block % 64)
@@ -220,7 +220,8 @@ int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
   for (int c = 0; c < n_chunks; ++c)
     if (chunk_iters[c] < 1) return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: chunk of < 1 iteration");
 
-  // reset the queue counters: row 0 holds every model's voxel count
+  // reset the queue counters: row 0 holds every model's voxel count (no
+  // reset kernel: the caller has done it on this stream)
   struct {
     int32_t first[8];
   } init;
@@ -228,8 +229,9 @@ int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
   for (int m = 0; m < nmodels; ++m) init.first[m] = (int32_t)jobs[m].a.N;
   int32_t total = (n_chunks + 1) * nmodels;
   void* rp[3] = {&counts, &total, &init};
-  PCA_HIP(hipModuleLaunchKernel(reset_kernel->fn, (uint32_t)((total + 255) / 256), 1, 1, 256, 1,
-                                1, 0, (hipStream_t)stream, rp, nullptr));
+  if (reset_kernel)
+    PCA_HIP(hipModuleLaunchKernel(reset_kernel->fn, (uint32_t)((total + 255) / 256), 1, 1, 256,
+                                  1, 1, 0, (hipStream_t)stream, rp, nullptr));
 
   // kernel argument block: jobs[nmodels], counts, chunk, it0, k, last
   const size_t jb = sizeof(pca_model_job) * (size_t)nmodels;
