@@ -116,7 +116,7 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
         }
       }
     }
-    Mdl::finish(P, V.p, s, q, dmp, V.g, V.h, V.hld);
+    Mdl::finish(V.p, s, q, dmp, V.g, V.h, V.hld);
   } else {
 #pragma unroll
     for (int j = 0; j < NP; ++j) {

@@ -354,7 +354,7 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
 
     # finish: apply the factors
     pf = _Printer(dag, dtype)
-    prods, l = [], 0
+    prods = []
     for j in range(n):
         for k in range(j + 1):
             prods.append(g.mul(jf[j][0], jf[k][0]))
@@ -389,9 +389,8 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
     out += ["  __device__ static __forceinline__ void accf(const Pre& P, "
             "const real* __restrict__ c, real y, real& f, real* s, real* q, real* dmp) {"]
     out += ind(acc) + ["  }"]
-    out += ["  __device__ static __forceinline__ void finish(const Pre& P, "
-            "const real* __restrict__ p, const real* s, const real* q, const real* dmp, "
-            "real* g, real* h, real* hld) {"]
+    out += ["  __device__ static __forceinline__ void finish(const real* __restrict__ p, "
+            "const real* s, const real* q, const real* dmp, real* g, real* h, real* hld) {"]
     out += ind(fin) + ["  }"]
     out += ["  __device__ static __forceinline__ void rprep(const real* __restrict__ p, RPre& R) {"]
     out += ind(rp) + ["  }"]
