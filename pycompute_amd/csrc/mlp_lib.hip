@@ -319,9 +319,11 @@ struct InferArgs {
 // concatenation of two consecutive 16-unit activation tiles: k slot 8g + i
 // holds unit 32c + 4g + i (i < 4) or 32c + 16 + 4g + i - 4 (i >= 4) of k
 // block c, and the weight fragments are gathered with the same permutation.
-// With LDSW the weights are staged once per workgroup into LDS in fragment
-// order (conflict-free 8- and 16-byte reads) and the grid is persistent.
-template <int H, int INP, int OUTP, int VT, bool LDSW, int OUTC>
+// LDSW stages weights once per workgroup into LDS in fragment order
+// (conflict-free 8- and 16-byte reads) on a persistent grid: 1 = all three
+// weight matrices (the 128-unit networks), 2 = W2 only (the 256-unit ones,
+// whose 128 KB W2 takes a workgroup of its own per CU); 0 = none.
+template <int H, int INP, int OUTP, int VT, int LDSW, int OUTC>
 __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   // outputs: OUTC when fixed at compile time (the per-output guards fold away), else a.OUT
   const int nout = OUTC > 0 ? OUTC : a.OUT;
@@ -332,33 +334,37 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   constexpr int V = 16 * VT;     // voxels per wavefront
   static_assert(V <= 64, "at most one voxel per lane in the epilogue");
   constexpr int W2OFF = H * INP, W3OFF = H * INP + H * H;
+  constexpr int W2L = LDSW == 1 ? W2OFF : 0;   // offset of W2 in wl
+  constexpr int WLN = LDSW == 1 ? W3OFF + OUTP * H : (LDSW == 2 ? H * H : 8);
   __shared__ float ybuf[4][OUTP * V];
-  __shared__ __attribute__((aligned(16))) short wl[LDSW ? W3OFF + OUTP * H : 8];
+  __shared__ __attribute__((aligned(16))) short wl[WLN];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int col = lane & 15;      // voxel within a tile
   const int kq = (lane >> 4) * 4; // k (or row) offset within a tile
-  if constexpr (LDSW) {
+  if constexpr (LDSW == 1) {
     stage_fragments<H, INP>(wl, a.w1, H);
-    stage_fragments_k32<H, H>(wl + W2OFF, a.w2, H);
     stage_fragments_k32<OUTP, H>(wl + W3OFF, a.w3, nout);
+  }
+  if constexpr (LDSW != 0) {
+    stage_fragments_k32<H, H>(wl + W2L, a.w2, H);
     __syncthreads();
   }
   // weight fragments: w1 (row tile r, k tile k); w2, w3 (row tile r, k block c)
   auto w1f = [&](int r, int k) -> half4 {
-    if constexpr (LDSW) return *(const half4*)(wl + ((r * KT1 + k) * 64 + lane) * 4);
+    if constexpr (LDSW == 1) return *(const half4*)(wl + ((r * KT1 + k) * 64 + lane) * 4);
     else return *(const half4*)(a.w1 + (long long)(r * 16 + col) * INP + k * 16 + kq);
   };
   auto w2f = [&](int r, int c) -> half8 {
-    if constexpr (LDSW) {
-      return *(const half8*)(wl + W2OFF + ((r * KC + c) * 64 + lane) * 8);
+    if constexpr (LDSW != 0) {
+      return *(const half8*)(wl + W2L + ((r * KC + c) * 64 + lane) * 8);
     } else {
       const short* p = a.w2 + (long long)(r * 16 + col) * H + c * 32 + kq;
       return cat8(*(const half4*)p, *(const half4*)(p + 16));
     }
   };
   auto w3f = [&](int r, int c) -> half8 {
-    if constexpr (LDSW) {
+    if constexpr (LDSW == 1) {
       return *(const half8*)(wl + W3OFF + ((r * KC + c) * 64 + lane) * 8);
     } else {
       half8 w = half8{0, 0, 0, 0, 0, 0, 0, 0};   // rows of W3 past OUT are zero
@@ -665,7 +671,8 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
   // weights in LDS when they fit next to the output buffer in 64 KB: then a
   // persistent grid of 512 workgroups (2 per CU) reuses the staged copy
   const bool ldsw = !big && OUTP == 16 && INP <= 32;
-  const int grid = (int)std::min<long long>((waves + 3) / 4, ldsw ? 512 : 8192);
+  // 256-unit networks stage W2 (128 KB) instead: one persistent workgroup per CU
+  const int grid = (int)std::min<long long>((waves + 3) / 4, ldsw ? 512 : (big ? 256 : 8192));
   hipStream_t s = (hipStream_t)stream;
 #define PCA_INFER(HH, II, OO, VT, L, OC) \
   hipLaunchKernelGGL((k_mlp_infer_t<HH, II, OO, VT, L, OC>), dim3(grid), dim3(256), 0, s, a)
@@ -679,19 +686,19 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
     else PCA_INFER(HH, II, OO, VT, L, 0);                                 \
   } while (0)
   if (!big) {
-    if (INP == 16 && OUTP == 16) PCA_INFER_OUT(128, 16, 16, 4, true);
-    else if (INP == 16) PCA_INFER(128, 16, 32, 4, false, 0);
-    else if (INP == 32 && OUTP == 16) PCA_INFER_OUT(128, 32, 16, 4, true);
-    else if (INP == 32) PCA_INFER(128, 32, 32, 4, false, 0);
-    else if (OUTP == 16) PCA_INFER(128, 64, 16, 4, false, 0);
-    else PCA_INFER(128, 64, 32, 4, false, 0);
+    if (INP == 16 && OUTP == 16) PCA_INFER_OUT(128, 16, 16, 4, 1);
+    else if (INP == 16) PCA_INFER(128, 16, 32, 4, 0, 0);
+    else if (INP == 32 && OUTP == 16) PCA_INFER_OUT(128, 32, 16, 4, 1);
+    else if (INP == 32) PCA_INFER(128, 32, 32, 4, 0, 0);
+    else if (OUTP == 16) PCA_INFER(128, 64, 16, 4, 0, 0);
+    else PCA_INFER(128, 64, 32, 4, 0, 0);
   } else {
-    if (INP == 16 && OUTP == 16) PCA_INFER(256, 16, 16, 2, false, 0);
-    else if (INP == 16) PCA_INFER(256, 16, 32, 2, false, 0);
-    else if (INP == 32 && OUTP == 16) PCA_INFER(256, 32, 16, 2, false, 0);
-    else if (INP == 32) PCA_INFER(256, 32, 32, 2, false, 0);
-    else if (OUTP == 16) PCA_INFER(256, 64, 16, 2, false, 0);
-    else PCA_INFER(256, 64, 32, 2, false, 0);
+    if (INP == 16 && OUTP == 16) PCA_INFER(256, 16, 16, 2, 2, 0);
+    else if (INP == 16) PCA_INFER(256, 16, 32, 2, 2, 0);
+    else if (INP == 32 && OUTP == 16) PCA_INFER(256, 32, 16, 2, 2, 0);
+    else if (INP == 32) PCA_INFER(256, 32, 32, 2, 2, 0);
+    else if (OUTP == 16) PCA_INFER(256, 64, 16, 2, 2, 0);
+    else PCA_INFER(256, 64, 32, 2, 2, 0);
   }
 #undef PCA_INFER_OUT
 #undef PCA_INFER
