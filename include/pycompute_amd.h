@@ -162,18 +162,23 @@ int pca_lm_launch(pca_kernel kernel, const pca_lm_args* args, const void* uconst
 /* Chunked fit with convergence compaction over `nmodels` models (kernels
  * "pca_lm_chunk" and "pca_reset_counts" of one generated module): resets the
  * queue counters (device array of (n_chunks+1)*nmodels int32) with a kernel,
- * or, when reset_kernel is NULL, relies on the caller having reset them on the
- * same stream (row 0 = each model's N, the rest 0; the module's
- * "pca_lm_reset_fit" does so together with the state of a single-model fit),
- * then launches one chunk per entry of chunk_iters (host array; their sum is
+ * or, when reset_kernel is NULL, relies on the caller having zeroed rows 1..
+ * n_chunks-1 on the same stream (a single chunk needs none: chunk 0 takes
+ * each model's N from its job), then launches one chunk per entry of
+ * chunk_iters (host array; their sum is
  * the iteration count) on a grid of `grid` blocks. Per-voxel results equal
  * those of pca_lm_launch with the same iteration count. Replaces the same
  * per-iteration launch loop (pycompute/cuda/lsqnonlin.py:1058-1073) and, for
- * several models, that many separate fits. Capturable in a graph. */
+ * several models, that many separate fits. Capturable in a graph.
+ * fresh_state != 0: the first chunk starts every voxel from the state
+ * SecondOrderLevenbergMarquardt.setup establishes (lambda 5, step_type 1,
+ * f 0, n_iter 0; lsqnonlin.py:1036-1056) instead of reading lam, step_type
+ * and f, so a captured whole fit needs no separate reset kernel; 0 resumes
+ * from the arrays. */
 int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
                           const pca_model_job* jobs, int32_t nmodels, int32_t* counts,
                           const int32_t* chunk_iters, int32_t n_chunks, uint32_t grid,
-                          uint32_t block, void* stream);
+                          uint32_t block, int32_t fresh_state, void* stream);
 
 /* hipGraph capture of everything launched on `stream` between begin and
  * end (no counterpart in the reference, which launches from a Python loop). */

@@ -85,7 +85,7 @@ def lib():
     L.pca_device_arch.argtypes = [ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32]
     L.pca_lm_chunked_launch.argtypes = [_P, _P, ctypes.POINTER(ModelJob), ctypes.c_int32, _P,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
-                                        ctypes.c_uint32, ctypes.c_uint32, _P]
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, _P]
     L.pca_graph_begin.argtypes = [_P]
     L.pca_graph_end.argtypes = [_P, ctypes.POINTER(_P)]
     L.pca_graph_launch.argtypes = [_P, _P]

@@ -171,19 +171,13 @@ class BatchedLM:
 
     def capture_graph(self, iters: int, tol=1e-8, mu=0.25, eta=0.75, acc=0.2, dec=2.0,
                       init=None):
-        """Capture reset_state(), optionally init(stream) (kernels writing the
-        initial guesses into the members' pars_t) and the whole chunked fit into
-        one hipGraph; replay it with replay_graph()."""
+        """Capture optionally init(stream) (kernels writing the initial guesses
+        into the members' pars_t) and the whole chunked fit, starting from the
+        setup() state as after reset_state(), into one hipGraph; replay it with
+        replay_graph()."""
         r = self._chunked(tol, mu, eta, acc, dec)
         r.set_iters(iters)
-        self.program.kernel("pca_lm_reset_state")
-
-        def prologue(stream):
-            self.reset_state(stream)
-            if init is not None:
-                init(stream)
-
-        r.capture(chunked.chunk_schedule(int(iters), self.chunk), prologue)
+        r.capture(chunked.chunk_schedule(int(iters), self.chunk), init, fresh=True)
         return r
 
     def replay_graph(self):

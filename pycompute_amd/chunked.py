@@ -85,9 +85,11 @@ class ChunkedRun:
         for i in range(len(self.jobs)):
             self.job_array[i].a.iters = int(iters)
 
-    def launch(self, schedule, stream=None, reset_counts=True):
-        """Launch the chunk sequence; reset_counts=False when the caller has
-        already reset the queue counters on this stream (pca_lm_reset_fit)."""
+    def launch(self, schedule, stream=None, reset_counts=True, fresh=False):
+        """Launch the chunk sequence. reset_counts=False: the counters are not
+        reset first (possible for a single chunk, which reads no counter).
+        fresh=True: the first chunk starts every voxel from the setup() state
+        instead of reading it."""
         if not schedule:
             return
         counts = self._counts_for(len(schedule))
@@ -97,14 +99,15 @@ class ChunkedRun:
         reset = self.program.kernel("pca_reset_counts") if reset_counts else None
         _lib.check(L.pca_lm_chunked_launch(self.program.kernel("pca_lm_chunk"), reset,
                                            self.job_array, len(self.jobs), counts.data_ptr(), k,
-                                           len(schedule), self.grid, self.block, s),
+                                           len(schedule), self.grid, self.block, int(fresh), s),
                    "pca_lm_chunked_launch")
 
     # -- hipGraph -------------------------------------------------------------
-    def capture(self, schedule, prologue=None, reset_counts=True):
+    def capture(self, schedule, prologue=None, fresh=False):
         """Capture prologue() (e.g. the init kernel) and the chunk sequence into
-        a graph on a side stream; returns the graph handle. reset_counts=False:
-        the prologue resets the queue counters itself."""
+        a graph on a side stream; returns the graph handle. fresh=True: every
+        replay starts the fit from the setup() state (no reset kernel); a
+        single chunk then needs no counter reset either."""
         self.destroy_graph()
         self._counts_for(len(schedule))
         self.program.kernel("pca_lm_chunk")
@@ -119,7 +122,8 @@ class ChunkedRun:
             try:
                 if prologue is not None:
                     prologue(h)
-                self.launch(schedule, stream=h, reset_counts=reset_counts)
+                self.launch(schedule, stream=h, reset_counts=not (fresh and len(schedule) == 1),
+                            fresh=fresh)
                 ok = True
             finally:
                 g = ctypes.c_void_p()

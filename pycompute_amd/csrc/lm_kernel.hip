@@ -390,6 +390,9 @@ struct pca_chunk_args {
   int it0;               // iteration index of the chunk's first iteration
   int k;                 // iterations in this chunk
   int last;              // 1: no chunk follows; every voxel stores its result
+  int fresh;             // 1: chunk 0 starts every voxel from the setup() state
+                         // (lambda 5, step_type 1, f 0, n_iter 0) instead of
+                         // reading it: a captured fit needs no reset kernel
 };
 
 namespace pca {
@@ -423,6 +426,12 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
     if (from_user) {
       v = e;
       load_user(a, v, V);
+      if (A.fresh) {
+        V.lam = (real)5;
+        V.st = 1;
+        V.f = V.f_first = (real)0;
+        if (a.n_iter) a.n_iter[v] = 0;
+      }
       if constexpr (per_voxel) {
 #pragma unroll
         for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * a.N + v];
@@ -536,7 +545,12 @@ __device__ __forceinline__ void dispatch(int m, const pca_chunk_args& A, long lo
 // ceil(count_m / 64) consecutive tasks, so every wavefront works on one model.
 extern "C" __global__ void __launch_bounds__(PCA_BLOCK) PCA_KATTR
 pca_lm_chunk(pca_chunk_args A) {
-  const int* cnt = A.counts + A.chunk * PCA_NMODELS;
+  // queue lengths: chunk 0 reads every model's voxels, later chunks the counts
+  // the previous chunk appended
+  int cnt[PCA_NMODELS];
+#pragma unroll
+  for (int m = 0; m < PCA_NMODELS; ++m)
+    cnt[m] = A.chunk == 0 ? (int)A.job[m].a.N : A.counts[A.chunk * PCA_NMODELS + m];
   long long tasks[PCA_NMODELS + 1];
   tasks[0] = 0;
 #pragma unroll
@@ -570,23 +584,6 @@ extern "C" __global__ void pca_reset_counts(int* counts, int n, pca_counts_init 
 extern "C" __global__ void pca_lm_reset_state(pca_lm_args a) {
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < a.N;
        v += (long long)gridDim.x * blockDim.x) {
-    a.lam[v] = (real)5;
-    a.step_type[v] = 1;
-    a.f[v] = (real)0;
-    if (a.n_iter) a.n_iter[v] = 0;
-  }
-}
-
-// Both resets of a single-model fit in one launch (a captured fit then starts
-// with one kernel fewer): the state of pca_lm_reset_state, and the queue
-// counters of pca_reset_counts for a sequence of n / PCA_NMODELS - 1 chunks.
-extern "C" __global__ void pca_lm_reset_fit(pca_lm_args a, int* counts, int n,
-                                            pca_counts_init init) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  for (long long i = t0; i < n; i += stride)
-    counts[i] = i < PCA_NMODELS ? init.first[i] : 0;
-  for (long long v = t0; v < a.N; v += stride) {
     a.lam[v] = (real)5;
     a.step_type[v] = 1;
     a.f[v] = (real)0;

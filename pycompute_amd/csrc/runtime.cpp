@@ -202,7 +202,7 @@ int pca_launch(pca_kernel kernel, uint32_t gx, uint32_t gy, uint32_t gz, uint32_
 int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
                           const pca_model_job* jobs, int32_t nmodels, int32_t* counts,
                           const int32_t* chunk_iters, int32_t n_chunks, uint32_t grid,
-                          uint32_t block, void* stream) {
+                          uint32_t block, int32_t fresh_state, void* stream) {
   if (!chunk_kernel || !jobs || !counts || !chunk_iters)
     return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: NULL argument");
   if (nmodels < 1 || nmodels > 4) return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: 1..4 models");
@@ -233,9 +233,10 @@ int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
     PCA_HIP(hipModuleLaunchKernel(reset_kernel->fn, (uint32_t)((total + 255) / 256), 1, 1, 256,
                                   1, 1, 0, (hipStream_t)stream, rp, nullptr));
 
-  // kernel argument block: jobs[nmodels], counts, chunk, it0, k, last
+  // kernel argument block: jobs[nmodels], counts, chunk, it0, k, last, fresh
+  // (zero padding to the struct's 8-byte alignment)
   const size_t jb = sizeof(pca_model_job) * (size_t)nmodels;
-  std::vector<unsigned char> buf(jb + sizeof(int32_t*) + 4 * sizeof(int32_t));
+  std::vector<unsigned char> buf(jb + sizeof(int32_t*) + 6 * sizeof(int32_t));
   std::memcpy(buf.data(), jobs, jb);
   std::memcpy(buf.data() + jb, &counts, sizeof(counts));
   int32_t* tail = (int32_t*)(buf.data() + jb + sizeof(int32_t*));
@@ -245,6 +246,7 @@ int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
     tail[1] = it0;
     tail[2] = chunk_iters[c];
     tail[3] = (c == n_chunks - 1) ? 1 : 0;
+    tail[4] = (c == 0 && fresh_state) ? 1 : 0;
     void* kp[1] = {buf.data()};
     PCA_HIP(hipModuleLaunchKernel(chunk_kernel->fn, grid, 1, 1, block, 1, 1, 0,
                                   (hipStream_t)stream, kp, nullptr));

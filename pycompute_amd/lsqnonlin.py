@@ -389,25 +389,16 @@ class SecondOrderLevenbergMarquardt(HipFunction):
 
     def capture_graph(self, iters: int, tol=1e-8, mu=0.25, eta=0.75, acc=0.2, dec=2.0,
                       init=None):
-        """Capture a whole fit -- reset_state() together with the queue
-        counters (one kernel), optionally init(stream) (a kernel writing the
-        initial guess into pars_t), and the chunked run -- into a hipGraph;
+        """Capture a whole fit -- optionally init(stream) (a kernel writing the
+        initial guess into pars_t), then the chunked run starting from the
+        setup() state (the first chunk takes lambda 5, step_type 1, f 0 and
+        n_iter 0 itself, as reset_state() would write them) -- into a hipGraph;
         replay it with replay_graph()."""
         if not self._use_chunked():
             raise RuntimeError("graph capture needs the chunked path (compaction=True)")
         r = self._chunked_run(tol, mu, eta, acc, dec)
         r.set_iters(iters)
-        a = self.lm_args(0, tol, mu, eta, acc, dec)
-        schedule = chunked.chunk_schedule(int(iters), self.chunk)
-        counts = r._counts_for(len(schedule))
-        self.program.kernel("pca_lm_reset_fit")
-
-        def prologue(stream):
-            launch_reset_fit(self.program, a, self.Nprobs, counts, len(schedule) + 1, stream)
-            if init is not None:
-                init(stream)
-
-        r.capture(schedule, prologue, reset_counts=False)
+        r.capture(chunked.chunk_schedule(int(iters), self.chunk), init, fresh=True)
         return r
 
     def replay_graph(self):
@@ -422,26 +413,6 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         F.launch(self.program, self.pars_t, self.data_t, self._cpv, self._ucbuf, self._ucbytes,
                  None, out, self.block)
         return out
-
-
-class CountsInit(ctypes.Structure):
-    _fields_ = [("first", ctypes.c_int32 * 8)]
-
-
-def launch_reset_fit(program, args, N, counts, n, stream):
-    """pca_lm_reset_fit: the state of launch_reset_state plus the n queue
-    counters of a single-model chunk sequence (row 0 = N, the rest 0)."""
-    grid = max(1, min((max(int(N), int(n)) + 255) // 256, 4096))
-    init = CountsInit()
-    init.first[0] = int(N)
-    cptr = ctypes.c_void_p(counts.data_ptr())
-    nn = ctypes.c_int32(int(n))
-    params = (ctypes.c_void_p * 4)(ctypes.cast(ctypes.pointer(args), ctypes.c_void_p),
-                                   ctypes.cast(ctypes.pointer(cptr), ctypes.c_void_p),
-                                   ctypes.cast(ctypes.pointer(nn), ctypes.c_void_p),
-                                   ctypes.cast(ctypes.pointer(init), ctypes.c_void_p))
-    _lib.check(_lib.lib().pca_launch(program.kernel("pca_lm_reset_fit"), grid, 1, 1, 256, 1,
-                                     1, 0, stream, params), "pca_lm_reset_fit")
 
 
 def launch_reset_state(program, args, N, stream):
