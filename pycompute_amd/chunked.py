@@ -74,8 +74,13 @@ class ChunkedRun:
         self._counts = None
         self._graph = None
         self._graph_key = None
+        self._graph_counts = None
 
     def _counts_for(self, n_chunks):
+        """Queue-length counters for a sequence of n_chunks chunks. A captured
+        graph holds the address of the counters it was recorded with, so a
+        longer launch after a capture gets a new buffer and the captured one
+        stays alive (self._graph_counts) until the graph is destroyed."""
         need = (n_chunks + 1) * len(self.jobs)
         if self._counts is None or self._counts.numel() < need:
             self._counts = torch.zeros(need, dtype=torch.int32, device=self.device)
@@ -133,6 +138,7 @@ class ChunkedRun:
         torch.cuda.current_stream(self.device).wait_stream(side)
         self._graph = g
         self._graph_key = tuple(schedule)
+        self._graph_counts = self._counts      # the buffer the graph addresses
         return g
 
     def replay(self, stream=None):
@@ -145,6 +151,7 @@ class ChunkedRun:
         if self._graph is not None:
             _lib.lib().pca_graph_destroy(self._graph)
             self._graph = None
+        self._graph_counts = None
 
     def __del__(self):
         try:

@@ -361,6 +361,10 @@ pca_lm_fit(pca_lm_args a, pca_uconsts uc) {
 // ---------------------------------------------------------------------------
 // chunked execution with convergence compaction
 // ---------------------------------------------------------------------------
+// Per-voxel consts travel with the working set only when they fit in
+// registers (at most PCA_CV_REGS values); the generator defines PCA_NO_CHUNK
+// for larger per-voxel layouts, which pca_lm_fit serves alone.
+#ifndef PCA_NO_CHUNK
 
 struct pca_ws {          // packed working set, entry e of capacity cap
   real* y;               // (M, cap)
@@ -410,8 +414,6 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
   const bool from_user = (A.chunk == 0);
 #ifdef PCA_CONSTS_PER_VOXEL
   constexpr bool per_voxel = true;
-  // Per-voxel consts travel with the working set only when they fit in
-  // registers; the host sends larger per-voxel layouts to pca_lm_fit instead.
   static_assert(NC * M <= PCA_CV_REGS, "per-voxel consts too large for the chunked kernel");
 #else
   constexpr bool per_voxel = false;
@@ -554,7 +556,7 @@ pca_lm_chunk(pca_chunk_args A) {
   long long tasks[PCA_NMODELS + 1];
   tasks[0] = 0;
 #pragma unroll
-  for (int m = 0; m < PCA_NMODELS; ++m) tasks[m + 1] = tasks[m] + (cnt[m] + 63) / 64;
+  for (int m = 0; m < PCA_NMODELS; ++m) tasks[m + 1] = tasks[m] + ((long long)cnt[m] + 63) / 64;
   const long long total = tasks[PCA_NMODELS];
   const long long waves_per_block = PCA_BLOCK / 64;
   const long long nwaves = (long long)gridDim.x * waves_per_block;
@@ -568,6 +570,8 @@ pca_lm_chunk(pca_chunk_args A) {
     pca::dispatch<0>(m, A, e, e < cnt[m]);
   }
 }
+
+#endif  // PCA_NO_CHUNK
 
 // counts[0:NMODELS] = first[0:NMODELS], the rest 0 (no memset in a captured graph)
 struct pca_counts_init {

@@ -17,6 +17,10 @@ from .program import BASE_OPTIONS, Module, read_csrc
 from .sym import hipgen
 
 DEFAULT_BLOCK = 256
+# per-voxel consts values a lane keeps in registers in the chunked kernel
+# (PCA_CV_REGS in csrc/lm_kernel.hip); larger per-voxel layouts run in the
+# single-launch kernel, which reads them from global memory
+CV_REGS = 64
 
 
 def dtype_name(dtype) -> str:
@@ -154,6 +158,8 @@ def build_source(spec: LMSpec, dag, members=()) -> str:
         head.append("#define PCA_REAL_DOUBLE 1")
     if spec.consts_per_voxel:
         head.append("#define PCA_CONSTS_PER_VOXEL 1")
+        if spec.nconst * spec.M > CV_REGS:
+            head.append("#define PCA_NO_CHUNK 1")
     if spec.fast_exp:
         head.append("#define PCA_FAST_EXP 1")
     if spec.strict:
@@ -190,16 +196,26 @@ def build_source(spec: LMSpec, dag, members=()) -> str:
 
 _programs: dict = {}
 _plock = threading.Lock()
+_key_locks: dict = {}
 
 
 def get_program(spec: LMSpec, members: tuple = ()) -> LMProgram:
+    """The program of ``spec`` (and batch ``members``), compiled once per
+    process; different specifications compile concurrently."""
     key = (spec, tuple(members))
     with _plock:
         p = _programs.get(key)
+        if p is not None:
+            return p
+        klock = _key_locks.setdefault(key, threading.Lock())
+    with klock:
+        with _plock:
+            p = _programs.get(key)
         if p is None:
             p = LMProgram(spec, tuple(members))
-            _programs[key] = p
-        return p
+            with _plock:
+                _programs[key] = p
+    return p
 
 
 # ---------------------------------------------------------------------------

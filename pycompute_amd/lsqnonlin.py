@@ -202,6 +202,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         # voxel-uniform consts, n > 8). None = use it whenever it applies.
         self.wide, self.tile = wide, int(tile)
         self._glm, self._glm_tab = None, None
+        # more consts than the kernel-argument block holds: per-voxel layout
+        self._uc_fits = self.nconst * self.ndata <= _lib.PCA_UC_MAX
         if self.expr_true:
             self._spec_uniform = self._spec(False)
             self.funcid = self._spec_uniform.funcid()
@@ -240,8 +242,12 @@ class SecondOrderLevenbergMarquardt(HipFunction):
     def get_deps(self):
         return []
 
-    def build(self, consts_per_voxel: bool = False):
-        """Generate and compile the module (eager, as lsqnonlin.py:973)."""
+    def build(self, consts_per_voxel: bool | None = None):
+        """Generate and compile the module (eager, as lsqnonlin.py:973). By
+        default the uniform-consts module, or the per-voxel one when nconst*M
+        exceeds the kernel-argument block."""
+        if consts_per_voxel is None:
+            consts_per_voxel = not self._uc_fits
         if not self.expr_true:
             self.program = self._plugin.program(consts_per_voxel)
             self.code = self.program.source
@@ -286,6 +292,12 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         else:
             self._uc, self._cpv = split_consts(consts_t, self.nconst, self.ndata, N,
                                                uniform_consts)
+            if self._uc is not None and not self._uc_fits:
+                # uniform values beyond the argument block: replicate them
+                self._cpv = torch.tensor(self._uc, dtype=self.dtype,
+                                         device=data_t.device)[:, None].expand(
+                                             -1, N).contiguous()
+                self._uc = None
         if (self._cpv is not None) != self.program_is_per_voxel():
             self.build(consts_per_voxel=self._cpv is not None)
         if self._uc is not None:
@@ -363,7 +375,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         if not self.compaction:
             return False
         # per-voxel consts ride along in the queues only when they fit in registers
-        return self._cpv is None or self.nconst * self.ndata <= 64
+        return self._cpv is None or self.nconst * self.ndata <= lm.CV_REGS
 
     def _chunked_run(self, tol, mu, eta, acc, dec):
         key = (float(tol), float(mu), float(eta), float(acc), float(dec))

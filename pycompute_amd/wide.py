@@ -115,6 +115,7 @@ def glm_flops(n: int, M: int, generic: dict) -> dict:
 
 _programs: dict = {}
 _lock = threading.Lock()
+_key_locks: dict = {}
 
 
 def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> GLMProgram | None:
@@ -126,7 +127,15 @@ def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> GLMProgram | N
         return None
     key = (spec, tile_max)
     with _lock:
-        if key not in _programs:
-            dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
-            _programs[key] = GLMProgram(spec, tile_max) if glm.glm_form(dag) else None
-        return _programs[key]
+        if key in _programs:
+            return _programs[key]
+        klock = _key_locks.setdefault(key, threading.Lock())
+    with klock:                 # different specifications compile concurrently
+        with _lock:
+            if key in _programs:
+                return _programs[key]
+        dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
+        prog = GLMProgram(spec, tile_max) if glm.glm_form(dag) else None
+        with _lock:
+            _programs[key] = prog
+        return prog
