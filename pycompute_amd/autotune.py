@@ -18,8 +18,8 @@ of the caller's own problem:
   * tile:          wide-model kernel register/LDS split (wide_candidates())
 
 The fastest candidate whose results agree with the reference-rounding build on
-the sample (final SSR within 1e-4 relative for at least as many voxels as the
-rounding noise of FP32 allows, checked against the first candidate) is kept.
+the sample (final SSR within 1e-4 relative on at least 98% of voxels, and no
+one-sided residual tail; see acceptable()) is kept.
 Results are cached as JSON next to the code-object cache.
 """
 from __future__ import annotations
@@ -112,14 +112,24 @@ def wide_candidates(allow_fast: bool = True):
     return out
 
 
+def acceptable(r: dict) -> bool:
+    """A rounding-level variant agrees with the reference-rounding build (the
+    first candidate) on nearly every voxel -- FP32 and FP64 runs of the
+    reference itself differ on about 1% -- and its disagreements are not
+    one-sided: voxels that end more than 1% worse may be at most twice as many
+    as those that end more than 1% better, plus 0.1% (a variant that only
+    rounds differently moves stopping points both ways; a broken one makes
+    residuals worse)."""
+    worse, better = r.get("worse", 0.0), r.get("better", 0.0)
+    return r["agree"] >= 0.98 and worse <= 2.0 * better + 1e-3
+
+
 def pick(results: list) -> dict:
-    """The winning entry of timed candidates ({"knobs", "ms", "agree"}, in
-    candidate order). Rounding-level variants must agree with the
-    reference-rounding build (the first candidate) on nearly every voxel --
-    FP32 and FP64 runs of the reference itself differ on about 1% -- and
+    """The winning entry of timed candidates ({"knobs", "ms", "agree",
+    "worse", "better"}, in candidate order): the fastest acceptable one, where
     timings within 1% of the fastest are a tie (run-to-run noise is of that
     order), won by the earliest candidate so that repeated tunings agree."""
-    ok = [r for r in results if r["agree"] >= 0.97]
+    ok = [r for r in results if acceptable(r)]
     fastest = min(r["ms"] for r in ok)
     return next(r for r in ok if r["ms"] <= fastest * 1.01)
 
@@ -155,14 +165,18 @@ def tune(make_solver, setup, model_expr: str, M: int, dtype: str, allow_fast: bo
         f = s.final_f()[0].double()
         if ref_f is None:
             ref_f = f
-            agree = 1.0
+            agree, worse, better = 1.0, 0.0, 0.0
         else:
-            rel = (f - ref_f).abs() / ref_f.abs().clamp_min(1e-30)
-            agree = float((rel <= 1e-4).double().mean())
+            rel = (f - ref_f) / ref_f.abs().clamp_min(1e-30)
+            agree = float((rel.abs() <= 1e-4).double().mean())
+            worse = float((rel > 1e-2).double().mean())
+            better = float((rel < -1e-2).double().mean())
         ms = min(ts)
-        results.append({"knobs": cand, "ms": ms, "agree": agree})
+        results.append({"knobs": cand, "ms": ms, "agree": agree, "worse": worse,
+                        "better": better})
         if log:
-            log(f"autotune {cand} {ms:.3f} ms agree {agree:.4f}")
+            log(f"autotune {cand} {ms:.3f} ms agree {agree:.4f} worse {worse:.5f} "
+                f"better {better:.5f}")
     best = pick(results)
     db[k] = {"best": best["knobs"], "ms": best["ms"], "results": results,
              "when": time.strftime("%Y-%m-%d %H:%M:%S")}

@@ -1,8 +1,15 @@
 """Mixed-model batched fits: several models' voxels in one persistent chunked
-launch sequence give the same per-voxel results as fitting each model alone."""
+launch sequence give the same per-voxel results as fitting each model alone,
+the strict batch equals the oracle's replay of every member bit for bit, and
+the full-size C3 volume (400k T2 + 400k IVIM + 200k stretched exponential)
+keeps those properties, with the oracle parity bars on a sample of each
+model."""
+import numpy as np
 import pytest
 import torch
 
+import oracle.genmodel as gm
+import parity
 from pycompute_amd import fit, synth
 from pycompute_amd.batch import BatchedLM
 from pycompute_amd.models import MODELS
@@ -94,3 +101,61 @@ def test_batch_graph_replay_and_resume(gpu):
         assert torch.equal(r2.members[i].f_t, ref.members[i].f_t), ORDER[i]
     f = ref.final_f()
     assert f.shape == (1, 5000) and bool(torch.isfinite(f).all())
+
+
+def test_strict_batch_members_equal_oracle(gpu):
+    """Each member of a strict batch equals the oracle's replay of that model
+    on the batch's own inputs, bit for bit (NaN matching NaN)."""
+    probs, p0 = _problems((1500, 2500, 1000), seed=13)
+    b = BatchedLM([(MODELS[k], p.M) for k, p in zip(ORDER, probs)], torch.float32, chunk=8,
+                  strict=True)
+    pars = [q.clone() for q in p0]
+    b.setup([(pars[i], p.consts_t, p.data_t, p.lb_t, p.ub_t) for i, p in enumerate(probs)])
+    b.run(100)
+    torch.cuda.synchronize()
+    for i, (p, m) in enumerate(zip(probs, b.members)):
+        mod = p.model
+        ref = gm.lm_run(mod.expr, mod.pars, mod.consts, p.consts, p.data_t.cpu().numpy(),
+                        p.lb_t.cpu().numpy(), p.ub_t.cpu().numpy(), p0[i].cpu().numpy(), 100)
+        got = {"params": m.pars_t, "lam": m.lam_t[0], "step_type": m.step_type_t[0],
+               "f": m.f_t[0], "first_f": m.first_f[0], "last_f": m.last_f[0],
+               "n_iter": m.n_iter_t[0]}
+        for k, t in got.items():
+            a = t.cpu().numpy()
+            same = (a == ref[k]) | (np.isnan(a) & np.isnan(ref[k])) if a.dtype.kind == "f" \
+                else (a == ref[k])
+            assert same.all(), (ORDER[i], k, int((~same).sum()))
+
+
+def test_full_size_mixed_batch(gpu):
+    """The C3 volume in one batched launch sequence with the fastest build:
+    identical to the three models fitted one by one with the same knobs,
+    parameters inside the bounds, final SSR finite, and each model's first
+    4,096 voxels within the oracle parity bars."""
+    sizes = (400_000, 400_000, 200_000)
+    probs, p0 = _problems(sizes, seed=7)
+    kw = dict(chunk=16, fast_exp=True, fast_div=True, reassoc=True, factored=True)
+    b = BatchedLM([(MODELS[k], p.M) for k, p in zip(ORDER, probs)], torch.float32, **kw)
+    pars = [q.clone() for q in p0]
+    b.setup([(pars[i], p.consts_t, p.data_t, p.lb_t, p.ub_t) for i, p in enumerate(probs)])
+    b.run(100)
+    torch.cuda.synchronize()
+    f = b.final_f()
+    assert bool(torch.isfinite(f).all())
+    ns = 4096
+    for i, (p, m) in enumerate(zip(probs, b.members)):
+        ref = _solo(p, p0[i], compaction=True, **kw)
+        torch.cuda.synchronize()
+        assert torch.equal(m.pars_t, ref.pars_t), ORDER[i]
+        assert torch.equal(m.n_iter_t, ref.n_iter_t), ORDER[i]
+        assert bool((m.pars_t >= p.lb_t).all()) and bool((m.pars_t <= p.ub_t).all())
+        its = m.n_iter_t.double().mean().item()
+        print(f"{ORDER[i]}: {its:.2f} iterations per voxel, converged "
+              f"{float((m.step_type_t == 0).double().mean()):.4f}", flush=True)
+        y = np.ascontiguousarray(p.data_t[:, :ns].cpu().numpy())
+        lb = np.ascontiguousarray(p.lb_t[:, :ns].cpu().numpy())
+        ub = np.ascontiguousarray(p.ub_t[:, :ns].cpu().numpy())
+        q0 = np.ascontiguousarray(p0[i][:, :ns].cpu().numpy())
+        r32, r64 = parity.oracle_pair(ORDER[i], p.consts, y, lb, ub, q0, 100)
+        parity.check(f"C3-{ORDER[i]}", ORDER[i], p.consts, y, m.pars_t[:, :ns].cpu().numpy(),
+                     r32, r64, gpu_first_f=m.first_f[0, :ns].cpu().numpy())

@@ -1,26 +1,19 @@
 """GPU parity of the fused HIP LM fitter against the CPU oracle (same inputs,
 same initial guess).
 
-Bars of the default (fast) build, checked per voxel:
-  * SSR at the initial guess (first_f) within 1e-4 relative for >= 99.9%;
-  * final SSR within 1e-4 relative for >= 98% of all voxels;
-  * voxels both sides converge (step_type 0): parameters within 1e-4 relative
-    (absolute floor 1e-7) for >= 85% of them, and the two sides agree on
-    convergence for >= 80% of voxels.
-These bars are set by FP32 rounding, not by the port: the FP32 oracle differs
-from the FP64 oracle by the same order (test_oracle.py and the bench's parity
-report).
-The accept/reject path amplifies last-bit differences (device expf vs the
-host's, FMA contraction) and IVIM's minimum is flat along f/D_1, so stopping
-points differ more in parameters than in SSR; the distributions are printed.
-Bit-exact agreement on every voxel is tested in the strict build
-(test_gpu_strict.py).
+The non-strict builds are held to the per-voxel bars of tests/parity.py,
+which are set against what FP32 itself does on the same inputs (the FP32
+and FP64 oracles): first_f, final SSR against the FP32 oracle, parameters
+against the FP64 optimum as close as the FP32 oracle's own, and a residual
+tail no heavier than the FP32 oracle's. Bit-exact agreement on every voxel
+is tested in the strict build (test_gpu_strict.py).
 """
 import numpy as np
 import pytest
 import torch
 
 import oracle
+import parity
 from pycompute_amd import fit, synth
 from pycompute_amd.models import IVIM, T2, SE
 
@@ -56,41 +49,34 @@ def _run_pair(model, w, iters, dtype=torch.float32, n=None, **kw):
     return prob, p0, gpu, ref
 
 
-def _check(gpu, ref, tag):
-    conv_g = gpu["step_type"] == 0
-    conv_r = ref["step_type"] == 0
-    agree = np.mean(conv_g == conv_r)
-    both = conv_g & conv_r
-    pr = _rel(gpu["params"][:, both], ref["params"][:, both], 1e-7).max(axis=0)
-    sr = _rel(gpu["final_f"], ref["final_f"], 1e-30)
-    print(f"[{tag}] N={gpu['step_type'].size} converged gpu {conv_g.mean():.4f} "
-          f"oracle {conv_r.mean():.4f} agree {agree:.4f}", flush=True)
-    _summary(f"[{tag}] param rel (both converged)", pr)
-    _summary(f"[{tag}] final SSR rel", sr)
-    fr = _rel(gpu["first_f"], ref["first_f"], 1e-30)
-    _summary(f"[{tag}] first_f rel", fr)
-    assert np.mean(fr < REL) >= 0.999, "first_f (SSR at the initial guess) disagrees"
-    assert agree >= 0.80
-    assert np.mean(pr < REL) >= 0.85
-    assert np.mean(sr < REL) >= 0.98
+def _check(prob, p0, gpu, tag, iters=100):
+    """The per-voxel bars of tests/parity.py against the FP32 and FP64
+    oracles run from the same start."""
+    y = prob.data_t.double().cpu().numpy().astype(np.float32)
+    lb, ub = prob.lb_t.cpu().numpy(), prob.ub_t.cpu().numpy()
+    r32, r64 = parity.oracle_pair(prob.model.name, prob.consts, y, lb.astype(np.float32),
+                                  ub.astype(np.float32), p0.cpu().numpy().astype(np.float32),
+                                  iters)
+    return parity.check(tag, prob.model.name, prob.consts, y, gpu["params"], r32, r64,
+                        gpu_first_f=gpu["first_f"])
 
 
 def test_ivim_c1_parity(gpu):
     w = synth.ivim_c1(4096, seed=1)
     prob, p0, g, r = _run_pair("ivim", w, 100)
-    _check(g, r, "ivim-c1")
+    _check(prob, p0, g, "ivim-c1")
 
 
 def test_ivim_c2_subset_parity(gpu):
     w = synth.ivim(65536, synth.B_C2, seed=2, snr=(20.0, 100.0))
     prob, p0, g, r = _run_pair("ivim", w, 100)
-    _check(g, r, "ivim-c2-64k")
+    _check(prob, p0, g, "ivim-c2-64k")
 
 
 def test_t2_and_se_parity(gpu):
     for model, w in (("t2", synth.t2(8192, seed=3)), ("se", synth.se(8192, seed=4))):
         prob, p0, g, r = _run_pair(model, w, 100)
-        _check(g, r, model)
+        _check(prob, p0, g, model)
 
 
 FAST_BUILDS = {"fast": dict(fast_exp=True, fast_div=True),
@@ -107,10 +93,10 @@ def test_fast_builds_parity(gpu, build):
     kw = FAST_BUILDS[build]
     w = synth.ivim(65536, synth.B_C2, seed=2, snr=(20.0, 100.0))
     prob, p0, g, r = _run_pair("ivim", w, 100, **kw)
-    _check(g, r, f"ivim-64k-{build}")
+    _check(prob, p0, g, f"ivim-64k-{build}")
     for model, wm in (("t2", synth.t2(8192, seed=3)), ("se", synth.se(8192, seed=4))):
         prob, p0, g, r = _run_pair(model, wm, 100, **kw)
-        _check(g, r, f"{model}-{build}")
+        _check(prob, p0, g, f"{model}-{build}")
 
 
 def _factored_vs_folded(prob, p0, sched_group, tag):
@@ -191,9 +177,18 @@ def test_factored_user_expressions(gpu, expr, pars, truth):
 
 
 def test_ivim_float64_parity(gpu):
+    """The FP64 build against the FP64 oracle: final SSR within 1e-10 and
+    parameters within 1e-6 on nearly every voxel (FP64 trajectories diverge
+    far more rarely than FP32 ones)."""
     w = synth.ivim_c1(4096, seed=7)
     prob, p0, g, r = _run_pair("ivim", w, 100, dtype=torch.float64)
-    _check(g, r, "ivim-f64")
+    sr = _rel(g["final_f"], r["final_f"], 1e-300)
+    pr = _rel(g["params"], r["params"], 1e-12).max(axis=0)
+    fr = _rel(g["first_f"], r["first_f"], 1e-300)
+    _summary("[ivim-f64] final SSR rel", sr)
+    _summary("[ivim-f64] param rel", pr)
+    assert np.mean(fr < 1e-12) >= 0.999
+    assert np.mean(sr < 1e-10) >= 0.99 and np.mean(pr < 1e-6) >= 0.98
 
 
 def test_default_init_matches_oracle(gpu):

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 import torch
 
+import parity
 from oracle import mlp as omlp
 from pycompute_amd import fit, synth
 from pycompute_amd import mlp
@@ -290,3 +291,85 @@ def test_mlp_initialised_graph_replays_equal_eager(gpu):
         assert bool(torch.isfinite(junk))
         for k in ("pars_t", "step_type_t", "lam_t", "f_t", "n_iter_t"):
             assert torch.equal(getattr(ref, k), getattr(s, k)), (rep, k)
+
+
+IVIM_BOX = ([0.8, 0.05, 5e-3, 0.5e-3], [1.2, 0.4, 0.1, 3e-3])
+FASTEST = dict(fast_exp=True, fast_div=True, reassoc=True, factored=True, waves_per_eu=2,
+               chunk=16)
+
+
+def _trained_ivim_net(samples=10_000_000, seed=0):
+    """The headline initialiser: 16 -> 128 -> 128 -> 4, trained on forward-model
+    signals of a separate seed (the benchmark's recipe)."""
+    net = mlp.ivim_mlp(synth.B_C2, hidden=128, seed=seed)
+    data, truth = net.synthesize(samples, seed=20240613, snr=(20.0, 100.0), lo=IVIM_BOX[0],
+                                 hi=IVIM_BOX[1])
+    net.fit_standardization(truth, data)
+    X, T = net.prepare(data, truth)
+    del data, truth
+    net.train(X, T, epochs=2, batch=8192)
+    return net
+
+
+def test_mlp_guess_lm_matches_oracle(gpu):
+    """C4 leg: LM refined from the network's guess (the fastest build, the
+    headline cap of 6 iterations) against the oracle run from the same guess,
+    on 65,536 voxels, to the parity bars of tests/parity.py."""
+    net = _trained_ivim_net(2_000_000)
+    w = synth.ivim(65536, synth.B_C2, seed=61, snr=(20.0, 100.0))
+    prob = fit.make_problem("ivim", w["consts"], w["data"])
+    p0 = torch.empty((4, prob.N), device="cuda")
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, p0, fallback=(prob.model, prob.consts))
+    s = fit.fit(prob, iters=6, params0=p0.clone(), **FASTEST)
+    torch.cuda.synchronize()
+    y = prob.data_t.cpu().numpy()
+    r32, r64 = parity.oracle_pair("ivim", prob.consts, y, prob.lb_t.cpu().numpy(),
+                                  prob.ub_t.cpu().numpy(), p0.cpu().numpy(), 6)
+    parity.check("C4 MLP guess + 6 LM iterations", "ivim", prob.consts, y,
+                 s.pars_t.cpu().numpy(), r32, r64, gpu_first_f=s.first_f.cpu().numpy()[0])
+
+
+def test_mlp_pipeline_4m_matched_residual(gpu):
+    """The C4 pipeline at full size: the network trained on 10M signals, the
+    iteration cap chosen on a held-out validation volume, then the
+    4,194,304-voxel volume fitted from the MLP guess at that cap reaches a
+    total final SSR within 1e-4 of plain LM's (default guess, 100
+    iterations), in far fewer iterations; the per-voxel ratio of the two
+    residuals is printed."""
+    net = _trained_ivim_net()
+    fb = lambda pr: (pr.model, pr.consts)     # noqa: E731
+    wv = synth.ivim(1 << 18, synth.B_C2, seed=99, snr=(20.0, 100.0))
+    pv = fit.make_problem("ivim", wv["consts"], wv["data"])
+    ref_v = float(fit.fit(pv, iters=100, **FASTEST).final_f().double().sum())
+    gv = torch.empty((4, pv.N), device="cuda")
+    net.infer(pv.data_t, pv.lb_t, pv.ub_t, gv, fallback=fb(pv))
+    cap = None
+    for c in (2, 3, 4, 5, 6, 8, 10, 12, 15, 20, 25, 30):
+        if float(fit.fit(pv, iters=c, params0=gv.clone(), **FASTEST).final_f().double().sum()) \
+                <= ref_v * (1 + 1e-4):
+            cap = c
+            break
+    assert cap is not None
+    del pv, gv
+    w = synth.ivim(1 << 22, synth.B_C2, seed=4, snr=(20.0, 100.0))
+    prob = fit.make_problem("ivim", w["consts"], w["data"])
+    del w
+    plain = fit.fit(prob, iters=100, **FASTEST)
+    f_plain = plain.final_f().double()[0]
+    it_plain = float(plain.n_iter_t.double().mean())
+    del plain
+    g = torch.empty((4, prob.N), device="cuda")
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, g, fallback=fb(prob))
+    s = fit.fit(prob, iters=cap, params0=g, **FASTEST)
+    f_mlp = s.final_f().double()[0]
+    it_mlp = float(s.n_iter_t.double().mean())
+    tot = float(f_mlp.sum()) / float(f_plain.sum()) - 1.0
+    ratio = (f_mlp / f_plain).cpu().numpy()
+    q = np.percentile(ratio, [0.1, 1, 50, 99, 99.9])
+    print(f"4M: cap {cap}, total SSR {tot:+.3e} vs plain, iterations {it_mlp:.2f} vs "
+          f"{it_plain:.2f}; per-voxel SSR_mlp/SSR_plain p0.1 p1 p50 p99 p99.9 = {q}",
+          flush=True)
+    assert tot <= 1e-4
+    assert it_mlp * 4 < it_plain
+    assert bool(torch.isfinite(s.pars_t).all())
+    assert bool((s.pars_t >= prob.lb_t).all()) and bool((s.pars_t <= prob.ub_t).all())

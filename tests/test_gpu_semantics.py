@@ -25,6 +25,7 @@ import torch
 
 import oracle
 import oracle.genmodel as gm
+import parity
 from pycompute_amd import fit, synth
 from pycompute_amd.lsqnonlin import F_GradF, SecondOrderLevenbergMarquardt
 from pycompute_amd.models import IVIM, T2
@@ -186,17 +187,6 @@ def test_hand_written_plugin_matches_oracle(gpu):
     torch.cuda.synchronize()
     lb, ub = prob.lb_t.cpu().numpy(), prob.ub_t.cpu().numpy()
     y = prob.data_t.cpu().numpy()
-    r32 = oracle.lm_run("t2", w["consts"], y, lb, ub, p0.cpu().numpy(), 100)
-    r64 = oracle.lm_run("t2", w["consts"], y, lb, ub, p0.cpu().numpy(), 100, dtype=np.float64)
-    Sg = oracle.ssr("t2", w["consts"], y, s.pars_t.cpu().numpy(), dtype=np.float64)
-    S32 = oracle.ssr("t2", w["consts"], y, r32["params"], dtype=np.float64)
-    S64 = oracle.ssr("t2", w["consts"], y, r64["params"], dtype=np.float64)
-    within = np.abs(Sg - S32) <= 1e-4 * S32
-    env = np.abs(Sg - S64) <= np.maximum(1e-4 * S64, np.abs(S32 - S64)) + 1e-12
-    print(f"hand-written T2 plugin: final SSR within 1e-4 of the FP32 oracle "
-          f"{within.mean():.5f}, FP64 envelope {env.mean():.5f}", flush=True)
-    assert within.mean() >= 0.99
-    assert env.mean() >= 0.99
-    pr = (np.abs(s.pars_t.cpu().numpy() - r64["params"]) /
-          np.abs(r64["params"])).max(axis=0)
-    assert np.median(pr) < 1e-4
+    r32, r64 = parity.oracle_pair("t2", w["consts"], y, lb, ub, p0.cpu().numpy(), 100)
+    parity.check("hand-written T2 plugin", "t2", w["consts"], y, s.pars_t.cpu().numpy(), r32,
+                 r64, gpu_first_f=s.first_f.cpu().numpy()[0])

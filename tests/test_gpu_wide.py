@@ -6,7 +6,9 @@ import pytest
 import torch
 
 import oracle
-from pycompute_amd import fit, synth
+import parity
+from pycompute_amd import fit, mlp, synth
+from pycompute_amd import init as _init
 from pycompute_amd.lsqnonlin import SecondOrderLevenbergMarquardt
 
 pytestmark = pytest.mark.gpu
@@ -95,3 +97,29 @@ def test_wide_linear_model_reaches_least_squares(gpu):
     # the reference's stop also needs the previous step accepted with rho > eta,
     # which FP32 noise in rho denies some voxels near the optimum
     assert float((s.step_type_t == 0).double().mean()) > 0.9
+
+
+def test_dki_mlp_leg_matches_oracle(gpu):
+    """C5 leg: the kurtosis network's guess (61 -> 256 -> 256 -> 22, trained on
+    forward-model signals of a separate seed), kept per voxel only where its
+    SSR beats the default guess, then the wide LM capped at 10 iterations,
+    against the oracle run from the same guesses, to the parity bars of
+    tests/parity.py."""
+    n = 2048
+    w, prob, pd = _dki(n, seed=62)
+    net = mlp.dki_mlp(w["consts"])
+    T = synth.dki_truth_gpu(1_000_000, seed=20240614)
+    data, truth = net.synthesize(1_000_000, seed=777, snr=(30.0, 30.0), truth=T)
+    net.fit_standardization(truth, data)
+    X, Tt = net.prepare(data, truth)
+    net.train(X, Tt, epochs=2, batch=8192)
+    p0 = pd.clone()
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, p0)
+    _init.keep_better(prob.model, prob.consts, prob.data_t, p0, pd)
+    s = fit.fit(prob, iters=10, params0=p0.clone(), wide=True)
+    torch.cuda.synchronize()
+    y = np.ascontiguousarray(w["data"])
+    r32, r64 = parity.oracle_pair("dki", w["consts"], y, prob.lb_t.cpu().numpy(),
+                                  prob.ub_t.cpu().numpy(), p0.cpu().numpy(), 10)
+    parity.check("C5 MLP guess + 10 wide-LM iterations", "dki", w["consts"], y,
+                 s.pars_t.cpu().numpy(), r32, r64, gpu_first_f=s.first_f.cpu().numpy()[0])

@@ -180,3 +180,11 @@ def test_autotune_pick_ties_and_agreement():
     assert autotune.pick(r)["knobs"] == {"k": 1}           # 0.5% slower: a tie, earlier wins
     r[1]["ms"] = 2.05
     assert autotune.pick(r)["knobs"] == {"k": 2}
+    # agreement just under 98%, or a one-sided residual tail, disqualifies
+    r[2]["agree"] = 0.975
+    assert autotune.pick(r)["knobs"] == {"k": 1}
+    r[2]["agree"] = 0.99
+    r[2].update(worse=0.004, better=0.001)              # 0.4% worse vs 0.1% better
+    assert autotune.pick(r)["knobs"] == {"k": 1}
+    r[2].update(worse=0.0025, better=0.001)             # within 2x + 0.1%
+    assert autotune.pick(r)["knobs"] == {"k": 2}
