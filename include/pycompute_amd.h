@@ -95,14 +95,12 @@ typedef struct pca_eval_args {
   int64_t N;
 } pca_eval_args;
 
-/* Packed working set of the chunked (compacting) fit: entry e of `cap`
- * entries, arrays (rows, cap) with the module's element type. */
+/* Packed queue of the chunked (compacting) fit: the voxels still iterating,
+ * entry e of `cap` entries, arrays (rows, cap) with the module's element
+ * type. Only the state that changes is queued; measurements, bounds and
+ * per-voxel consts are read from the caller's arrays through idx. */
 typedef struct pca_ws {
-  void* y;          /* (M, cap) measurements */
-  void* cv;         /* (nconst*M, cap) per-voxel consts, or NULL */
-  void* p;          /* (n, cap) */
-  void* lo;         /* (n, cap) */
-  void* hi;         /* (n, cap) */
+  void* p;          /* (n, cap) parameters */
   void* lam;        /* (cap) */
   void* f;          /* (cap) */
   void* f_first;    /* (cap) */

@@ -128,7 +128,7 @@ class BatchedLM:
             mem.last_f = torch.zeros((1, N), **kw)
             mem.n_iter_t = torch.zeros((1, N), dtype=torch.int32, device=dev)
             if mem.ws is None or mem.ws.cap < max(N, 1):
-                mem.ws = chunked.Workset(N, s.M, s.nparam, 0, self.dtype, dev)
+                mem.ws = chunked.Workset(N, s.nparam, self.dtype, dev)
         if self._run is not None:
             self._run.destroy_graph()
         self._run, self._key = None, None

@@ -29,18 +29,18 @@ def chunk_schedule(iters: int, k: int) -> list:
 
 
 class Workset:
-    """Two packed queues (ping-pong) for one model's voxels."""
+    """Two packed queues (ping-pong) for one model's voxels. A queue entry is
+    the state that changes (parameters, lambda, f, first f, step type,
+    iteration count) and the voxel's index; the kernel reads the voxel's data,
+    bounds and per-voxel consts from the caller's arrays through the index."""
 
-    def __init__(self, N, M, n, ncm_per_voxel, dtype, device):
+    def __init__(self, N, n, dtype, device):
         self.cap = max(int(N), 1)
         kw = dict(dtype=dtype, device=device)
         self.tensors = []
         self.structs = []
         for _ in range(2):
-            t = {"y": torch.empty((M, self.cap), **kw),
-                 "cv": torch.empty((ncm_per_voxel, self.cap), **kw) if ncm_per_voxel else None,
-                 "p": torch.empty((n, self.cap), **kw), "lo": torch.empty((n, self.cap), **kw),
-                 "hi": torch.empty((n, self.cap), **kw), "lam": torch.empty(self.cap, **kw),
+            t = {"p": torch.empty((n, self.cap), **kw), "lam": torch.empty(self.cap, **kw),
                  "f": torch.empty(self.cap, **kw), "f_first": torch.empty(self.cap, **kw),
                  "idx": torch.empty(self.cap, dtype=torch.int32, device=device),
                  "cnt": torch.empty(self.cap, dtype=torch.int32, device=device),

@@ -19,7 +19,10 @@
 // pca_lm_chunk: the same iteration, run as a sequence of chunks of k
 // iterations over compacted work queues (one per model). After each chunk a
 // wavefront appends its still-active voxels to the next queue with one atomic
-// per wave; converged voxels write their results and leave. Waves therefore
+// per wave; converged voxels write their results and leave (a queue entry
+// holds only the state that changes -- parameters, lambda, f, step type,
+// iteration count -- and the voxel index, through which the data and bounds
+// are read from the caller's arrays). Waves therefore
 // stay full of active voxels instead of idling behind the slowest lane, and
 // several models (different n, M) share one persistent launch. Per-voxel
 // results are identical to pca_lm_fit.
@@ -153,20 +156,17 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
   for (int j = 0; j < NP; ++j)
     if (pca_nonfinite(V.s[j])) V.s[j] = 0;
 
-  // projected-gradient convergence (lsqnonlin.py:691-729)
+  // projected-gradient convergence (lsqnonlin.py:691-729); selects rather
+  // than branches (a NaN compares false both ways and stays unclamped)
   {
     bool clamped = false;
     real norm = 0;
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
       real t = V.p[j] - V.g[j];
-      if (t > V.hi[j]) {
-        clamped = true;
-        t = V.hi[j];
-      } else if (t < V.lo[j]) {
-        clamped = true;
-        t = V.lo[j];
-      }
+      const bool above = t > V.hi[j], below = t < V.lo[j];
+      clamped = clamped | above | below;
+      t = above ? V.hi[j] : (below ? V.lo[j] : t);
       t = V.p[j] - t;
       norm += t * t;
     }
@@ -232,31 +232,20 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
 #pragma unroll
   for (int i = 0; i < NP; ++i) pred += V.s[i] * V.g[i];
   const real rho = actual / pred;
-  if (rho > S.mu && actual > 0) {
-#pragma unroll
-    for (int j = 0; j < NP; ++j) V.p[j] = ptp[j];
-    if (rho > S.eta) {
-      V.lam *= S.acc;
-      V.st = 1;
-    } else {
-      V.st = 2;
-    }
-  } else {
-    V.lam *= S.dec;
-    V.st = 4;
-  }
-  if (pred < 0) {
-    V.lam *= S.dec;
-    V.st |= 8;
-  }
+  // accept / reject and the damping update, as selects
+  const bool accept = rho > S.mu && actual > 0;
+  const bool good = accept && rho > S.eta;
+  const bool nondesc = pred < 0;
+  real lam = good ? V.lam * S.acc : (accept ? V.lam : V.lam * S.dec);
+  lam = nondesc ? lam * S.dec : lam;
+  V.lam = lam;
+  V.st = (good ? 1 : (accept ? 2 : 4)) | (nondesc ? 8 : 0);
 
-  // clamp (lsqnonlin.py:584-607)
+  // take the trial point and clamp it (lsqnonlin.py:584-607)
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
-    if (V.p[j] > V.hi[j])
-      V.p[j] = V.hi[j];
-    else if (V.p[j] < V.lo[j])
-      V.p[j] = V.lo[j];
+    const real q = accept ? ptp[j] : V.p[j];
+    V.p[j] = q > V.hi[j] ? V.hi[j] : (q < V.lo[j] ? V.lo[j] : q);
   }
 }
 
@@ -366,12 +355,8 @@ pca_lm_fit(pca_lm_args a, pca_uconsts uc) {
 // for larger per-voxel layouts, which pca_lm_fit serves alone.
 #ifndef PCA_NO_CHUNK
 
-struct pca_ws {          // packed working set, entry e of capacity cap
-  real* y;               // (M, cap)
-  real* cv;              // (NC*M, cap) per-voxel consts, or null
-  real* p;               // (NP, cap)
-  real* lo;              // (NP, cap)
-  real* hi;              // (NP, cap)
+struct pca_ws {          // packed queue of the voxels still iterating, entry e of cap:
+  real* p;               // (NP, cap) parameters
   real* lam;             // (cap)
   real* f;               // (cap)
   real* f_first;         // (cap)
@@ -379,7 +364,8 @@ struct pca_ws {          // packed working set, entry e of capacity cap
   int* cnt;              // (cap) iterations executed in this run
   signed char* st;       // (cap)
   long long cap;
-};
+};                       // data, bounds and per-voxel consts are read from the
+                         // caller's arrays through idx (they never change)
 
 struct pca_model_job {
   pca_lm_args a;         // the caller's arrays for this model
@@ -439,15 +425,15 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
         for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * a.N + v];
       }
     } else {
-      const long long cap = in.cap;
+      const long long cap = in.cap, N = a.N;
       v = in.idx[e];
 #pragma unroll
-      for (int i = 0; i < M; ++i) V.y[i] = in.y[(long long)i * cap + e];
+      for (int i = 0; i < M; ++i) V.y[i] = a.data[(long long)i * N + v];
 #pragma unroll
       for (int j = 0; j < NP; ++j) {
         V.p[j] = in.p[(long long)j * cap + e];
-        V.lo[j] = in.lo[(long long)j * cap + e];
-        V.hi[j] = in.hi[(long long)j * cap + e];
+        V.lo[j] = a.lb[(long long)j * N + v];
+        V.hi[j] = a.ub[(long long)j * N + v];
       }
       V.lam = in.lam[e];
       V.f = in.f[e];
@@ -456,7 +442,7 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
       cnt_total = in.cnt[e];
       if constexpr (per_voxel) {
 #pragma unroll
-        for (int q = 0; q < NCV; ++q) cv[q] = in.cv[(long long)q * cap + e];
+        for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * N + v];
       }
     }
   }
@@ -488,23 +474,13 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
     const long long pos = base + __popcll(below);
     const long long cap = out.cap;
 #pragma unroll
-    for (int i = 0; i < M; ++i) out.y[(long long)i * cap + pos] = V.y[i];
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      out.p[(long long)j * cap + pos] = V.p[j];
-      out.lo[(long long)j * cap + pos] = V.lo[j];
-      out.hi[(long long)j * cap + pos] = V.hi[j];
-    }
+    for (int j = 0; j < NP; ++j) out.p[(long long)j * cap + pos] = V.p[j];
     out.lam[pos] = V.lam;
     out.f[pos] = V.f;
     out.f_first[pos] = V.f_first;
     out.st[pos] = (signed char)V.st;
     out.cnt[pos] = cnt_total;
     out.idx[pos] = (int)v;
-    if constexpr (per_voxel) {
-#pragma unroll
-      for (int q = 0; q < NCV; ++q) out.cv[(long long)q * cap + pos] = cv[q];
-    }
   } else if (valid) {
     store_result(a, v, V, cnt_total, cnt > 0);
   }

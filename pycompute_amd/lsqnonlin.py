@@ -383,8 +383,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             return self._chunk_state[1]
         dev = self.data_t.device
         if self._ws is None:
-            ncm = self.nconst * self.ndata if self._cpv is not None else 0
-            self._ws = chunked.Workset(self.Nprobs, self.ndata, self.nparam, ncm, self.dtype, dev)
+            self._ws = chunked.Workset(self.Nprobs, self.nparam, self.dtype, dev)
             if self._uc is not None:
                 self._uc_t = torch.tensor(self._uc, dtype=self.dtype, device=dev)
         a = self.lm_args(0, tol, mu, eta, acc, dec)
