@@ -19,7 +19,8 @@ same inputs by the FP32 and FP64 oracles:
     less two points, as the FP32 oracle's own parameters are;
   * residual tail: for t in (1e-4, 1e-3, 1e-2, 1e-1), the fraction of voxels
     whose final SSR exceeds the FP64 oracle's by more than t (relative) is at
-    most twice the FP32 oracle's fraction plus 0.03%, and the largest excess
+    most twice the FP32 oracle's fraction plus 0.03% and two voxels, and the
+    largest excess
     is at most three times the FP32 oracle's (floor 1e-2).
 """
 from __future__ import annotations
@@ -81,9 +82,10 @@ def check(tag, model, consts, y, gpu_params, r32, r64, gpu_first_f=None, min_agr
         assert out["first_f_ok"] >= 0.999, (tag, "first_f", out["first_f_ok"])
     assert agree >= min_agree, (tag, "final SSR vs FP32 oracle", agree)
     assert par_g >= par_32 - 0.02, (tag, "parameters vs FP64", par_g, par_32)
+    slack = 3e-4 + 2.0 / eg.size           # two voxels of a small sample
     for t in TAILS:
         mg, m32 = float(np.mean(eg > t)), float(np.mean(e32 > t))
         out[f"tail_gt_{t:g}"] = (mg, m32)
-        assert mg <= 2 * m32 + 3e-4, (tag, "tail", t, mg, m32)
+        assert mg <= 2 * m32 + slack, (tag, "tail", t, mg, m32)
     assert eg.max() <= max(3 * e32.max(), 1e-2), (tag, "max excess", eg.max(), e32.max())
     return out
