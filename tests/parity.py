@@ -20,8 +20,7 @@ same inputs by the FP32 and FP64 oracles:
   * residual tail: for t in (1e-4, 1e-3, 1e-2, 1e-1), the fraction of voxels
     whose final SSR exceeds the FP64 oracle's by more than t (relative) is at
     most twice the FP32 oracle's fraction plus 0.03% and two voxels, and the
-    largest excess
-    is at most three times the FP32 oracle's (floor 1e-2).
+    largest excess is at most three times the FP32 oracle's (floor 1e-2).
 """
 from __future__ import annotations
 
