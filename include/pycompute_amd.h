@@ -178,6 +178,32 @@ int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
                           const int32_t* chunk_iters, int32_t n_chunks, uint32_t grid,
                           uint32_t block, int32_t fresh_state, void* stream);
 
+/* Bytes of the file-scope device array behind pca_lm_chunked_launch_static,
+ * per device (96 MiB). */
+#define PCA_STATIC_SCRATCH_BYTES ((size_t)96 << 20)
+int64_t pca_static_scratch_bytes(void);
+
+/* The chunked, compacting fit of pca_lm_chunked_launch with no allocation:
+ * the queue counters and both work queues of every model are carved from a
+ * file-scope device array of PCA_STATIC_SCRATCH_BYTES (per device, held until
+ * the process ends) instead of caller memory, and the counters are cleared by
+ * a kernel at the start of every call. No counterpart in the reference; it is
+ * for callers that cannot allocate (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The caller passes its inputs and
+ * outputs: one pca_lm_args per model (as in pca_model_job.a), the device
+ * pointer of each model's voxel-uniform consts (or NULL), each model's
+ * parameter count, and is_double for the element type. Two queues of N
+ * entries per model must fit: 2 * N * ((n + 3) * es + 9) bytes plus 256-byte
+ * alignment per array (4-parameter float32 IVIM: about 1.36M voxels per
+ * call); larger sizes return PCA_ERR_LIMIT before anything is launched.
+ * Calls overlapping on one device share the array and race. Results equal
+ * pca_lm_chunked_launch's with the same chunk schedule. */
+int pca_lm_chunked_launch_static(pca_kernel chunk_kernel, const pca_lm_args* args,
+                                 const void* const* uconsts, const int32_t* nparam,
+                                 int32_t nmodels, int32_t is_double, const int32_t* chunk_iters,
+                                 int32_t n_chunks, uint32_t grid, uint32_t block,
+                                 int32_t fresh_state, void* stream);
+
 /* hipGraph capture of everything launched on `stream` between begin and
  * end (no counterpart in the reference, which launches from a Python loop). */
 int pca_graph_begin(void* stream);

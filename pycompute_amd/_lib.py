@@ -85,13 +85,20 @@ def lib():
     L.pca_lm_chunked_launch.argtypes = [_P, _P, ctypes.POINTER(ModelJob), ctypes.c_int32, _P,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, _P]
+    L.pca_static_scratch_bytes.restype = ctypes.c_int64
+    L.pca_static_scratch_bytes.argtypes = []
+    L.pca_lm_chunked_launch_static.argtypes = [
+        _P, ctypes.POINTER(LMArgs), ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int32),
+        ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, _P]
     L.pca_graph_begin.argtypes = [_P]
     L.pca_graph_end.argtypes = [_P, ctypes.POINTER(_P)]
     L.pca_graph_launch.argtypes = [_P, _P]
     L.pca_graph_destroy.argtypes = [_P]
     for name in ("pca_compile", "pca_module_load", "pca_module_unload", "pca_module_kernel",
                  "pca_lm_launch", "pca_ssr_launch", "pca_eval_launch", "pca_launch",
-                 "pca_device_arch", "pca_lm_chunked_launch", "pca_graph_begin",
+                 "pca_device_arch", "pca_lm_chunked_launch", "pca_lm_chunked_launch_static",
+                 "pca_graph_begin",
                  "pca_graph_end", "pca_graph_launch", "pca_graph_destroy"):
         getattr(L, name).restype = ctypes.c_int
     if L.pca_abi_version() != ABI_VERSION:
