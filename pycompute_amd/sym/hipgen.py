@@ -204,6 +204,9 @@ class _Factorer:
         return n.op != "num" and self.fs(n) <= self.pars
 
     def split(self, n: E.Node):
+        """A quotient a/b is split as a * (1/b_1) * (1/b_2) ..., b_i the
+        factors of b, so that a parameter-only divisor becomes a per-iteration
+        reciprocal and a const-only one joins the consts factor."""
         coef, P, K, U = 1.0, [], [], []
         stack = [n]
         while stack:
@@ -215,6 +218,17 @@ class _Factorer:
                 stack.append(x.args[0])
             elif x.op == "mul":
                 stack.extend(x.args)
+            elif x.op == "div" and not (x.args[1].op == "num" and x.args[1].value == 0.0):
+                stack.append(x.args[0])
+                dc, dP, dK, dU = self.split(x.args[1])
+                if dc == 0.0:
+                    U.append(x)
+                    continue
+                coef /= dc
+                one = self.g.num(1)
+                for group, dst in ((dP, P), (dK, K), (dU, U)):
+                    for d in group:
+                        dst.append(self.g.div(one, d))
             else:
                 f = self.fs(x)
                 (P if f <= self.pars else K if f <= self.consts else U).append(x)
@@ -258,6 +272,8 @@ class _MeasPrinter(_Printer):
     def _children(self, n: E.Node):
         if n.op in ("num", "sym") or self.fac.param_only(n):
             return ()
+        if n.op == "div" and self.fac.param_only(n.args[1]):
+            return (n.args[0],)
         if n.op == "fn" and n.name == "exp":
             coef, P, K, U = self.fac.split(n.args[0])
             if K:
@@ -280,6 +296,11 @@ class _MeasPrinter(_Printer):
                     if node.id in self._expk:
                         k, x = self._expk[node.id]
                         txt = f"PCA_EXPK({self.ref(k)}, {self.ref(x)})"
+                    elif node.op == "div" and self.fac.param_only(node.args[1]):
+                        # a parameter-only divisor: multiply by its reciprocal,
+                        # computed once per iteration in the table
+                        inv = self.fac.g.div(self.fac.g.num(1), node.args[1])
+                        txt = f"{self.ref(node.args[0])}*{self.ref(inv)}"
                     else:
                         txt = self._format(node, [self.ref(a) for a in node.args])
                     self.names[node.id] = self._tmp(txt)
