@@ -119,7 +119,7 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
         }
       }
     }
-    Mdl::finish(V.p, s, q, dmp, V.g, V.h, V.hld);
+    Mdl::finish(V.p, P.L, s, q, dmp, V.g, V.h, V.hld);
   } else {
 #pragma unroll
     for (int j = 0; j < NP; ++j) {

@@ -92,18 +92,19 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
         B[tri(i, j)] /= (sc[i] * sc[j]);
     }
 
-  // GMW81
+  // GMW81. The running maxima are fmax: every candidate is an absolute value
+  // and the maxima start at 0, so fmax equals the reference's compare-and-
+  // assign bit for bit, a NaN candidate included (both keep the old maximum).
   real gamma = 0, xi = 0;
 #pragma unroll
   for (int i = 0; i < n; ++i)
 #pragma unroll
     for (int j = 0; j <= i; ++j) {
       const real t = pca_abs(B[tri(i, j)]);
-      if (i == j) {
-        if (t > gamma) gamma = t;
-      } else {
-        if (t > xi) xi = t;
-      }
+      if (i == j)
+        gamma = pca_fmax(gamma, t);
+      else
+        xi = pca_fmax(xi, t);
     }
   if (n > 1) {
     if constexpr (FAST)
@@ -112,8 +113,7 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
       xi /= pca_sqrt((real)(n * n - 1));
   }
   real beta2 = PCA_EPS_GMW, delta = PCA_EPS_GMW;
-  if (beta2 < gamma) beta2 = gamma;
-  if (beta2 < xi) beta2 = xi;
+  beta2 = pca_fmax(pca_fmax(beta2, gamma), xi);
   if (gamma + xi > (real)1) delta *= (gamma + xi);
   real rbeta2 = 0, rd[n];
   if constexpr (FAST) rbeta2 = pca_rcp_fast(beta2);
@@ -136,10 +136,7 @@ __device__ __forceinline__ void gmw81_solve(const real* h, const real* hld, cons
     }
     real theta = 0;
 #pragma unroll
-    for (int i = j + 1; i < n; ++i) {
-      const real t = pca_abs(B[tri(i, j)]);
-      if (theta < t) theta = t;
-    }
+    for (int i = j + 1; i < n; ++i) theta = pca_fmax(theta, pca_abs(B[tri(i, j)]));
     theta *= theta;
     real d = pca_abs(B[tri(j, j)]);
     if (d < delta) d = delta;

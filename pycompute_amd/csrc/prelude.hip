@@ -79,5 +79,14 @@ __device__ __forceinline__ float pca_rcp_fast(float a) { return __builtin_amdgcn
 __device__ __forceinline__ double pca_rcp_fast(double a) { return 1.0 / a; }
 __device__ __forceinline__ float pca_sqrt_fast(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double pca_sqrt_fast(double a) { return sqrt(a); }
+// max(L*uu, 1e-4) - L*uu = max(1e-4 - L*uu, 0): the damping floor's excess
+// over L*uu (factored accumulation). For float one FMA with the clamp
+// modifier: the value lies below 1e-4, so clamping to [0, 1] is the max.
+__device__ __forceinline__ float pca_floor_excess(float L, float uu) {
+  return __builtin_amdgcn_fmed3f(__builtin_fmaf(-L, uu, PCA_DAMP_FLOOR), 0.0f, 1.0f);
+}
+__device__ __forceinline__ double pca_floor_excess(double L, double uu) {
+  return fmax(__builtin_fma(-L, uu, PCA_DAMP_FLOOR), 0.0);
+}
 __device__ __forceinline__ bool pca_nonfinite(float a) { return __builtin_isnan(a) || __builtin_isinf(a); }
 __device__ __forceinline__ bool pca_nonfinite(double a) { return __builtin_isnan(a) || __builtin_isinf(a); }

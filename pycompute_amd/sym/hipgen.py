@@ -324,6 +324,11 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
         g_j  = A_j  * sum u_j r
         h_jk = A_j A_k * sum u_j u_k + A''_jk * sum u''_jk r
         hld_j = sum max(L_j u_j^2, 1e-4),  L_j = lam A_j^2
+              = L_j sum u_j^2 + sum max(1e-4 - L_j u_j^2, 0)
+    The damping keeps the reference's per-measurement floor; in the second
+    form the per-measurement part is one clamped FMA (its value lies in
+    [0, 1e-4], so the clamp to [0, 1] is the max with 0) and the sum of
+    u_j^2 is the Gauss-Newton sum already accumulated for h_jj.
     Sums over the same u are shared (a second derivative often has the u of a
     Jacobian entry). The model m = A_m u_m gives r = fma(A_m, u_m, -y). Equal
     to accum() up to rounding."""
@@ -360,7 +365,7 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
         for k in range(j + 1):
             if j == k:
                 acc.append(f"{{ const real uu = {uj}*{uj}; q[{l}] += uu; "
-                           f"dmp[{j}] += pca_fmax(P.L[{j}]*uu, {floor}); }}")
+                           f"dmp[{j}] += pca_floor_excess(P.L[{j}], uu); }}")
             else:
                 acc.append(f"q[{l}] = pca_fma({uj}, {pm.ref(jf[k][1])}, q[{l}]);")
             l += 1
@@ -383,7 +388,8 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
     fin = list(pf.lines)
     for j in range(n):
         fin.append(f"g[{j}] = {pf.ref(jf[j][0])}*s[{sidx[jf[j][1].id][0]}];")
-        fin.append(f"hld[{j}] = dmp[{j}];")
+        jj = j * (j + 1) // 2 + j
+        fin.append(f"hld[{j}] = pca_fma(L[{j}], q[{jj}], dmp[{j}]);")
     for l in range(nh):
         gn = f"{pf.ref(prods[l])}*q[{l}]"
         if hf[l] is None:
@@ -411,7 +417,8 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
             "const real* __restrict__ c, real y, real& f, real* s, real* q, real* dmp) {"]
     out += ind(acc) + ["  }"]
     out += ["  __device__ static __forceinline__ void finish(const real* __restrict__ p, "
-            "const real* s, const real* q, const real* dmp, real* g, real* h, real* hld) {"]
+            "const real* L, const real* s, const real* q, const real* dmp, real* g, real* h, "
+            "real* hld) {"]
     out += ind(fin) + ["  }"]
     out += ["  __device__ static __forceinline__ void rprep(const real* __restrict__ p, RPre& R) {"]
     out += ind(rp) + ["  }"]

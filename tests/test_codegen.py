@@ -210,15 +210,18 @@ def test_factored_split_is_exact(expr, pars, consts):
         np.testing.assert_allclose(au, x, rtol=1e-12, atol=1e-300)
 
 
+@pytest.mark.parametrize("lam", [1e-6, 0.37, 1e3])
 @pytest.mark.parametrize("expr,pars,consts", FACTOR_EXPRS)
-def test_factored_sums_equal_fghhl(expr, pars, consts):
+def test_factored_sums_equal_fghhl(expr, pars, consts, lam):
     """The factored sums (g = A sum u r, h = A_j A_k sum u_j u_k + A'' sum u'' r,
-    hld = sum max(lam A^2 u^2, 1e-4) + h_jj) reproduce the fghhl loop of the
-    reference (lsqnonlin.py:237-256) in float64."""
+    hld = lam A^2 sum u^2 + sum max(1e-4 - lam A^2 u^2, 0) + h_jj) reproduce
+    the fghhl loop of the reference (lsqnonlin.py:237-256) in float64, with
+    the damping floor active on every measurement (lam 1e-6), on some and on
+    none (lam 1e3)."""
     dag = hipgen.res_jac_hes(expr, pars, consts)
     fac = hipgen._Factorer(dag)
     rng = np.random.default_rng(6)
-    M, lam = 12, 0.37
+    M = 12
     p = [rng.uniform(0.3, 1.5) for _ in dag.pars]
     cs = [rng.uniform(0.2, 2.0, M) for _ in dag.consts]
     y = rng.uniform(0.0, 2.0, M)
@@ -250,7 +253,10 @@ def test_factored_sums_equal_fghhl(expr, pars, consts):
                 v += float(np.ravel(ev(hf[l][0]))[0]) * np.sum(ev(hf[l][1]) * r)
             h.append(v)
             if j == k:
-                hld[j] = np.sum(np.maximum(lam * A[j] ** 2 * u[j] ** 2, 1e-4)) + v
+                # the emitted form: L sum u^2 + sum max(1e-4 - L u^2, 0)
+                L = lam * A[j] ** 2
+                uu = u[j] ** 2
+                hld[j] = L * np.sum(uu) + np.sum(np.maximum(1e-4 - L * uu, 0.0)) + v
             l += 1
     np.testing.assert_allclose(g, g_ref, rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(h, h_ref, rtol=1e-10, atol=1e-12)
