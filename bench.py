@@ -35,6 +35,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _q(x, qs=(50, 99, 99.9, 100)):
+    import numpy as np
+    x = np.asarray(x, np.float64)
+    x = x[np.isfinite(x)]
+    v = np.percentile(x, qs) if x.size else [float("nan")] * len(qs)
+    return {("max" if q == 100 else f"p{q:g}"): float(a) for q, a in zip(qs, v)}
+
+
+def fp64_error(model, consts, y, lb, ub, p0, iters, gpu_params, threads=None):
+    """The GPU result against the FP64 oracle run from the same start on a
+    sample: relative excess of the final SSR over the FP64 oracle's
+    (positive = the GPU ends worse), relative parameter error, and the FP32
+    oracle's own figures on the same sample for scale."""
+    import numpy as np
+    import oracle
+    d = np.float64
+    y = np.ascontiguousarray(y)
+    args = dict(nthreads=threads or 0)
+    r64 = oracle.lm_run(model, consts, y.astype(d), lb.astype(d), ub.astype(d), p0.astype(d),
+                        iters, dtype=d, **args)
+    r32 = oracle.lm_run(model, consts, y, lb, ub, p0, iters, **args)
+    s64 = oracle.ssr(model, consts, y.astype(d), r64["params"], dtype=d, **args)
+    sg = oracle.ssr(model, consts, y.astype(d), np.asarray(gpu_params, d), dtype=d, **args)
+    s32 = oracle.ssr(model, consts, y.astype(d), r32["params"], dtype=d, **args)
+    pe = lambda p: (np.abs(np.asarray(p, d) - r64["params"]) /
+                    np.maximum(np.abs(r64["params"]), 1e-7)).max(axis=0)
+    return {"sample": int(y.shape[1]), "iters": iters,
+            "final_ssr_excess_vs_fp64": _q((sg - s64) / np.abs(s64)),
+            "final_ssr_abs_rel_vs_fp64": _q(np.abs(sg - s64) / np.abs(s64)),
+            "params_rel_vs_fp64": _q(pe(gpu_params)),
+            "params_within_1e-4_of_fp64": float(np.mean(pe(gpu_params) < 1e-4)),
+            "fp32_oracle": {"final_ssr_excess_vs_fp64": _q((s32 - s64) / np.abs(s64)),
+                            "params_within_1e-4_of_fp64": float(np.mean(pe(r32["params"]) < 1e-4))}}
+
+
 def cpu_baseline(w, p0, lb, ub, gpu_params, gpu_ssr, sample, min_seconds=10.0):
     """Oracle (FP32 restatement, OpenMP) on a sample of the workload."""
     import numpy as np
@@ -63,16 +98,31 @@ def cpu_baseline(w, p0, lb, ub, gpu_params, gpu_ssr, sample, min_seconds=10.0):
     conv = (ref["step_type"] == 0)
     rel_p = (np.abs(gpu_params[:, :sample] - ref["params"]) /
              np.maximum(np.abs(ref["params"]), 1e-7)).max(axis=0)
+    p64 = ref64["params"]
+    pe = lambda p: (np.abs(np.asarray(p, np.float64) - p64) /
+                    np.maximum(np.abs(p64), 1e-7)).max(axis=0)
+    eg = (gpu_ssr[:sample].astype(np.float64) - ssr64) / np.abs(ssr64)
+    e32 = (ssr.astype(np.float64) - ssr64) / np.abs(ssr64)
+    tails = {f"{t:g}": {"gpu": float(np.mean(eg > t)), "fp32_oracle": float(np.mean(e32 > t))}
+             for t in (1e-4, 1e-3, 1e-2, 1e-1)}
     return ({"value": sample * runs / el, "unit": "voxel-fits/s", "cores": threads,
              "kind": "port",
              "sample": f"first {sample} voxels of the same workload, {runs} full fits "
-                       f"(FP32 restatement of the reference iteration, OpenMP)"},
+                       f"(FP32 restatement of the reference iteration, OpenMP)",
+             "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (oracle/Makefile: "
+                      "built in one container and loaded on another machine's cores, so "
+                      "no -march=native; contraction off to match the strict GPU build)"},
             {"sample": sample,
              "final_ssr_rel_le_1e-4_vs_fp32_oracle": float(np.mean(rel_f <= 1e-4)),
              "final_ssr_rel_p99_vs_fp32_oracle": float(np.percentile(rel_f, 99)),
              "final_ssr_rel_le_1e-4_vs_fp64_oracle": float(np.mean(rel_f64 <= 1e-4)),
              "fp32_oracle_vs_fp64_oracle_le_1e-4": float(np.mean(rel_o <= 1e-4)),
              "params_rel_le_1e-4_oracle_converged": float(np.mean(rel_p[conv] <= 1e-4)),
+             "params_within_1e-4_of_fp64": float(np.mean(pe(gpu_params[:, :sample]) < 1e-4)),
+             "fp32_oracle_params_within_1e-4_of_fp64": float(np.mean(pe(ref["params"]) < 1e-4)),
+             "final_ssr_excess_vs_fp64": _q(eg),
+             "fp32_oracle_final_ssr_excess_vs_fp64": _q(e32),
+             "tail_fraction_excess_over_fp64_gt": tails,
              "oracle_converged": float(conv.mean())})
 
 
@@ -80,19 +130,31 @@ IVIM_BOX = ([0.8, 0.05, 5e-3, 0.5e-3], [1.2, 0.4, 0.1, 3e-3])   # training param
 ITER_CAPS = (2, 3, 4, 5, 6, 8, 10, 12, 15, 20, 25, 30, 40, 50, 70, 100)
 
 
+def kernel_source_sha() -> str:
+    """First 16 hex digits of the sha256 of csrc/lm_kernel.hip (the queue
+    layout the traffic records were measured with)."""
+    import hashlib
+    with open(os.path.join(ROOT, "pycompute_amd", "csrc", "lm_kernel.hip"), "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()[:16]
+
+
 def _traffic(knobs):
     """HBM bytes per fit of the chunked LM kernel, from the copy-calibrated
     FETCH_SIZE / WRITE_SIZE counter runs recorded in profiles/ for the same
-    knob set (None when no record has these knobs)."""
+    chunk length and kernel source (None when no record matches). The traffic
+    is set by the chunk schedule and the queue layout; block size and
+    occupancy hint moved it by under 0.3% where both were measured."""
     path = os.path.join(ROOT, "profiles", "pca_lm_chunk_ivim_1M.traffic.json")
     try:
         with open(path) as fh:
             recs = json.load(fh)
     except (OSError, ValueError):
         return None
-    k = {key: v for key, v in knobs.items() if key != "keep_workspace"}
+    sha = kernel_source_sha()
     for rec in (recs if isinstance(recs, list) else [recs]):
-        if rec.get("tuning") == k:
+        t = rec.get("tuning", {})
+        if (rec.get("kernel_source_sha") == sha and t.get("chunk") == knobs.get("chunk")
+                and t.get("max_chunks", 0) == knobs.get("max_chunks", 0)):
             return rec["bytes_per_fit"]
     return None
 
@@ -110,7 +172,7 @@ def _time_replays(replay, args):
     return 1e3 * (time.perf_counter() - t0) / args.steps
 
 
-def run_mlp(args, prob, knobs, plain_ssr, stream):
+def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w=None):
     """Train the initialiser on forward-model signals (separate seed), choose
     the LM iteration cap on a held-out validation volume (the smallest cap whose
     total final SSR is within 1e-4 of plain LM's), then time the whole
@@ -193,6 +255,22 @@ def run_mlp(args, prob, knobs, plain_ssr, stream):
     f = solver.final_f().double()[0]
     ssr = float(f.sum())
     it = float(solver.n_iter_t.double().mean())
+    extra = {}
+    if plain_f is not None:
+        ratio = (f / plain_f.double()).cpu().numpy()
+        extra["per_voxel_ssr_ratio_mlp_over_plain"] = _q(
+            ratio, (0.1, 1, 50, 99, 99.9, 100))
+        extra["per_voxel_ssr_ratio_mlp_over_plain"]["frac_le_1_plus_1e-4"] = float(
+            (ratio <= 1 + 1e-4).mean())
+    if w is not None and not args.no_cpu:
+        # the FP64 oracle from the same (network) guess on a sample
+        ns = min(16384, N)
+        g = torch.empty((4, N), dtype=torch.float32, device=prob.data_t.device)
+        net.infer(prob.data_t, prob.lb_t, prob.ub_t, g, fallback=(prob.model, prob.consts))
+        extra["fp64_oracle_error"] = fp64_error(
+            "ivim", w["consts"], w["data"][:, :ns], prob.lb_t[:, :ns].cpu().numpy(),
+            prob.ub_t[:, :ns].cpu().numpy(), g[:, :ns].cpu().numpy(), cap,
+            solver.pars_t[:, :ns].cpu().numpy())
     # component times: MLP inference alone, the mean of 20 back-to-back launches
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, fallback=(prob.model, prob.consts))
@@ -235,6 +313,9 @@ def run_mlp(args, prob, knobs, plain_ssr, stream):
             "lm_frac_of_fp32_peak": lm_flops / (lm_ms * 1e-3) / PEAK_FP32,
             "validation": {"voxels": pv.N, "seed": 99, "plain_total_ssr": ref_ssr},
             "default_init_capped": default_capped,
+            "training_break_even_volumes": (train_s / ((plain_ms - ms) * 1e-3)
+                                            if plain_ms and plain_ms > ms else None),
+            **extra,
             "_net": net, "_cap": cap}
 
 
@@ -317,6 +398,33 @@ def run_mixed(args, stream):
     ms_solo = timed(lambda: [s.replay_graph() for s in solo])
     same = all(torch.equal(m.pars_t, s.pars_t) for m, s in zip(b.members, solo))
     log(f"mixed batch {ms_batch:.3f} ms, separately {ms_solo:.3f} ms, identical {same}")
+    # per model: iterations, LM kernel time alone, algorithmic rate, FP64 error
+    per_model = {}
+    for name, p, q, s, m, f in zip(order, probs, p0, solo, b.members, flops):
+        ts = []
+        for _ in range(3):
+            s.pars_t.copy_(q)
+            s.reset_state()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            s.run(100)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        it_m = int(m.n_iter_t.sum())
+        cv_m = int((m.step_type_t == 0).sum())
+        alg_m = (it_m - cv_m) * f["F_it"] + cv_m * f["F_conv"]
+        per_model[name] = {"voxels": p.N, "iterations_per_voxel": it_m / p.N,
+                           "converged_frac": cv_m / p.N, "flops_per_iteration": f["F_it"],
+                           "lm_kernel_ms_alone": min(ts),
+                           "ns_per_voxel_iteration_alone": min(ts) * 1e6 / it_m,
+                           "lm_frac_of_fp32_peak_alone": alg_m / (min(ts) * 1e-3) / PEAK_FP32}
+        if not args.no_cpu:
+            ns = 4096
+            per_model[name]["fp64_oracle_error"] = fp64_error(
+                name, p.consts, p.data_t[:, :ns].cpu().numpy(), p.lb_t[:, :ns].cpu().numpy(),
+                p.ub_t[:, :ns].cpu().numpy(), q[:, :ns].cpu().numpy(), 100,
+                m.pars_t[:, :ns].cpu().numpy())
     return {"workload": "400k T2 (32 echoes) + 400k IVIM (16 b) + 200k stretched exp (16 b), "
                         "Rician SNR 40, default init, max 100 iterations",
             "voxels": ntot, "value": ntot / (ms_batch * 1e-3), "unit": "voxel-fits/s",
@@ -326,7 +434,8 @@ def run_mixed(args, stream):
             "identical_to_separate": same,
             "iterations_per_voxel": sum(its) / ntot,
             "lm_kernel_ms": k_ms, "lm_tflops": alg / (k_ms * 1e-3) / 1e12,
-            "lm_frac_of_fp32_peak": alg / (k_ms * 1e-3) / PEAK_FP32}
+            "lm_frac_of_fp32_peak": alg / (k_ms * 1e-3) / PEAK_FP32,
+            "per_model": per_model}
 
 
 def run_ivim_4m(args, knobs, net, cap):
@@ -341,6 +450,8 @@ def run_ivim_4m(args, knobs, net, cap):
     N = prob.N
     out = {"workload": "IVIM 4,194,304 voxels, 16 b-values 0-800, SNR U[20,100]",
            "voxels": N}
+    finals = {}
+    ns = 8192
     for name, iters, use_mlp in (("plain_lm", 100, False), ("mlp_lm", cap, True)):
         s = fit.make_solver(prob, compaction=True, **knobs)
         pars = torch.empty((4, N), dtype=torch.float32, device=prob.data_t.device)
@@ -356,10 +467,24 @@ def run_ivim_4m(args, knobs, net, cap):
 
         s.capture_graph(iters, init=init)
         ms = _time_replays(s.replay_graph, args)
+        finals[name] = s.final_f().double()[0]
         out[name] = {"iter_cap": iters, "ms_per_step": ms, "value": N / (ms * 1e-3),
-                     "total_final_ssr": float(s.final_f().double().sum()),
+                     "total_final_ssr": float(finals[name].sum()),
                      "iterations_per_voxel": float(s.n_iter_t.double().mean())}
+        if not args.no_cpu:
+            g = torch.empty((4, ns), dtype=torch.float32, device=prob.data_t.device)
+            sub = fit.make_problem("ivim", w["consts"], w["data"][:, :ns])
+            if use_mlp:
+                net.infer(sub.data_t, sub.lb_t, sub.ub_t, g, fallback=(sub.model, sub.consts))
+            else:
+                fit._init.default_init(sub.model, sub.consts, sub.data_t, sub.lb_t, sub.ub_t,
+                                       out=g)
+            out[name]["fp64_oracle_error"] = fp64_error(
+                "ivim", w["consts"], w["data"][:, :ns], sub.lb_t.cpu().numpy(),
+                sub.ub_t.cpu().numpy(), g.cpu().numpy(), iters, s.pars_t[:, :ns].cpu().numpy())
         del s
+    ratio = (finals["mlp_lm"] / finals["plain_lm"]).cpu().numpy()
+    out["per_voxel_ssr_ratio_mlp_over_plain"] = _q(ratio, (0.1, 1, 50, 99, 99.9, 100))
     out["speedup"] = out["plain_lm"]["ms_per_step"] / out["mlp_lm"]["ms_per_step"]
     out["total_ssr_rel_vs_plain"] = (out["mlp_lm"]["total_final_ssr"] /
                                      out["plain_lm"]["total_final_ssr"] - 1.0)
@@ -506,13 +631,32 @@ def run_dki(args, stream):
                           "inference_tflops": mlp_flops / (infer_ms * 1e-3) / 1e12})
     log(f"DKI: plain {ms_plain:.2f} ms, MLP-init {res['mlp_lm']['ms_per_step']:.2f} ms "
         f"(cap {caps['mlp']}), default capped {res['default_init_capped']['ms_per_step']:.2f} ms")
+    res["mlp_gain_over_default_capped"] = (res["default_init_capped"]["ms_per_step"] /
+                                           res["mlp_lm"]["ms_per_step"])
+    fp64 = {}
+    if not args.no_cpu:
+        # the FP64 oracle on a sample, from the same guesses
+        ns = 1024
+        sub = fit.make_problem("dki", w["consts"], np.ascontiguousarray(w["data"][:, :ns]))
+        lbs, ubs = sub.lb_t.cpu().numpy(), sub.ub_t.cpu().numpy()
+        for name, iters, guess in (("plain_lm", 100, None), ("mlp_lm", caps["mlp"], "mlp")):
+            g = fit.default_init(sub)
+            if guess:
+                gd = g.clone()
+                net.infer(sub.data_t, sub.lb_t, sub.ub_t, g)
+                _init.keep_better(sub.model, sub.consts, sub.data_t, g, gd)
+            r = fit.fit(sub, iters=iters, params0=g.clone(), **kn)
+            fp64[name] = fp64_error("dki", w["consts"], w["data"][:, :ns], lbs, ubs,
+                                    g.cpu().numpy(), iters, r.pars_t.cpu().numpy())
+    res["mlp_lm"]["fp64_oracle_error"] = fp64.get("mlp_lm")
     return {"workload": "DKI 22 parameters, 2,097,152 voxels, 61 measurements, SNR 30",
             "voxels": N, "kernel": "pca_lm_glm (wide-model kernel)", "tuning": kn,
             "plain_lm": {"ms_per_step": ms_plain, "value": N / (ms_plain * 1e-3),
                          "total_final_ssr": plain_ssr, "iterations_per_voxel": its / N,
                          "lm_kernel_ms": k_ms, "lm_tflops": tflops,
                          "lm_frac_of_fp32_peak": tflops * 1e12 / PEAK_FP32,
-                         "excluded_voxels": int((~ok).sum())},
+                         "excluded_voxels": int((~ok).sum()),
+                         "fp64_oracle_error": fp64.get("plain_lm")},
             **res, "speedup_vs_plain_lm": ms_plain / res["mlp_lm"]["ms_per_step"]}
 
 
@@ -638,11 +782,26 @@ def main():
         torch.cuda.synchronize()
         pk.append(e0.elapsed_time(e1))
 
-    plain_ssr = float(solver.final_f().double().sum())
+    plain_f = solver.final_f()[0]
+    plain_ssr = float(plain_f.double().sum())
     plain_value = N / (ms_step * 1e-3)
+    # the strict build (reproducible exp/log/pow, no FMA contraction: what the
+    # oracle replays bit for bit) on the same fit, for the cost of exact parity
+    strict = make(compaction=True, strict=True, chunk=knobs["chunk"], block=knobs["block"])
+    sk = []
+    for _ in range(3):
+        restore(strict)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        strict.run(100)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        sk.append(e0.elapsed_time(e1))
+    del strict
     value, ms_value, mlp_info = plain_value, ms_step, None
     if not args.plain_only:
-        mlp_info = run_mlp(args, prob, knobs, plain_ssr, stream)
+        mlp_info = run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=plain_f,
+                           plain_ms=ms_step, w=w)
         value, ms_value = mlp_info.pop("value"), mlp_info.pop("ms_per_step")
     net, cap = (mlp_info.pop("_net"), mlp_info.pop("_cap")) if mlp_info else (None, None)
     ivim_4m = None if (args.no_4m or net is None) else run_ivim_4m(args, knobs, net, cap)
@@ -665,7 +824,8 @@ def main():
                    else "default segmented log-linear", "tuning": knobs,
                    "execution": "hipGraph: reset + init + chunked LM with compaction"},
         "plain_lm": {"value": plain_value, "ms_per_step": ms_step, "total_final_ssr": plain_ssr,
-                     "iterations_per_voxel": it_sum / N},
+                     "iterations_per_voxel": it_sum / N,
+                     "strict_build_lm_kernel_ms": min(sk)},
         "mlp_lm": mlp_info,
         "speedup_vs_plain_lm": (value / plain_value) if mlp_info else None,
         "ivim_4m": ivim_4m,

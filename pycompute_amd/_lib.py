@@ -16,7 +16,7 @@ import torch  # noqa: F401  (must be loaded before the HIP library, see above)
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libpycompute_amd.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 PCA_UC_MAX = 256
 
@@ -43,8 +43,9 @@ class EvalArgs(ctypes.Structure):
 
 
 class WS(ctypes.Structure):
-    _fields_ = [("p", _P), ("lam", _P), ("f", _P), ("f_first", _P), ("idx", _P), ("cnt", _P),
-                ("st", _P), ("cap", ctypes.c_int64)]
+    _fields_ = [("y", _P), ("cv", _P), ("lo", _P), ("hi", _P), ("p", _P), ("lam", _P),
+                ("f", _P), ("f_first", _P), ("idx", _P), ("cnt", _P), ("st", _P),
+                ("cap", ctypes.c_int64)]
 
 
 class ModelJob(ctypes.Structure):

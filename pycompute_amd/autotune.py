@@ -6,6 +6,8 @@ of the caller's own problem:
 
   * block:         threads per workgroup (wavefronts per workgroup = block/64)
   * chunk:         iterations between compactions of the work queue
+  * max_chunks:    at most this many chunks (0 = no limit): the last one takes
+                   the remaining iterations
   * unroll:        measurement-loop unroll (0 = full; a rolled loop puts the
                    per-voxel data arrays in scratch, so full wins for small n)
   * fast_exp / fast_div: v_exp_f32 and the approximate FP32 divide (only when
@@ -32,8 +34,9 @@ import torch
 
 from . import program
 
-DEFAULT = {"block": 256, "chunk": 16, "unroll": 0, "fast_exp": False, "fast_div": False,
-           "reassoc": False, "waves_per_eu": 0, "factored": False, "sched_group": 0}
+DEFAULT = {"block": 256, "chunk": 16, "max_chunks": 0, "unroll": 0, "fast_exp": False,
+           "fast_div": False, "reassoc": False, "waves_per_eu": 0, "factored": False,
+           "sched_group": 0}
 
 
 def _cache_path() -> str:
@@ -74,11 +77,11 @@ def candidates(nparam: int, allow_fast: bool):
                  (True, True, True, 0, False, 0), (True, True, True, 2, False, 0),
                  (True, True, True, 2, True, 0), (True, True, True, 0, True, 4)]
     for fe, fd, ra, wpe, fac, sg in fast:
-        for chunk in (8, 16, 24):
+        for chunk, mc in ((8, 0), (16, 0), (24, 0), (16, 4)):
             for block in (256, 128):
-                out.append({"block": block, "chunk": chunk, "unroll": 0, "fast_exp": fe,
-                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe,
-                            "factored": fac, "sched_group": sg})
+                out.append({"block": block, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                            "waves_per_eu": wpe, "factored": fac, "sched_group": sg})
     return out
 
 
@@ -91,11 +94,11 @@ def batch_candidates(allow_fast: bool = True):
         [(True, True, False, False), (True, True, True, False), (True, True, True, True)]
         if allow_fast else [])
     for fe, fd, ra, fac in fast:
-        for chunk in (8, 16):
+        for chunk, mc in ((8, 0), (16, 0), (16, 4)):
             for wpe in (0, 2):
-                out.append({"block": 256, "chunk": chunk, "unroll": 0, "fast_exp": fe,
-                            "fast_div": fd, "reassoc": ra, "waves_per_eu": wpe,
-                            "factored": fac, "sched_group": 0})
+                out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                            "waves_per_eu": wpe, "factored": fac, "sched_group": 0})
     return out
 
 

@@ -68,10 +68,11 @@ class BatchedLM:
     """models: list of (model, ndata), model an Expr, a models.ModelSpec or an
     (expr, pars, consts) triple; at most 4 models. Consts must be
     voxel-uniform (one protocol per model). Knobs as SecondOrderLevenbergMarquardt's
-    chunked path: block, chunk, unroll, fast_exp, fast_div, strict."""
+    chunked path: block, chunk, max_chunks, unroll, fast_exp, fast_div, strict."""
 
     def __init__(self, models, dtype=torch.float32, block: int = lm.DEFAULT_BLOCK,
-                 chunk: int = 8, fast_exp: bool = False, fast_div: bool = False,
+                 chunk: int = 8, max_chunks: int = 0, fast_exp: bool = False,
+                 fast_div: bool = False,
                  strict: bool = False, unroll: int = 0, waves_per_eu: int = 0,
                  reassoc: bool = False, sched_group: int = 0, factored: bool = False,
                  compaction: bool = True, **_):
@@ -79,7 +80,7 @@ class BatchedLM:
             raise ValueError("a batch holds 1 to 4 models")
         self.dtype_name = lm.dtype_name(dtype)
         self.dtype = lm.torch_dtype(self.dtype_name)
-        self.block, self.chunk = int(block), int(chunk)
+        self.block, self.chunk, self.max_chunks = int(block), int(chunk), int(max_chunks)
         specs = []
         for m, ndata in models:
             e = _as_expr(m)
@@ -128,7 +129,7 @@ class BatchedLM:
             mem.last_f = torch.zeros((1, N), **kw)
             mem.n_iter_t = torch.zeros((1, N), dtype=torch.int32, device=dev)
             if mem.ws is None or mem.ws.cap < max(N, 1):
-                mem.ws = chunked.Workset(N, s.nparam, self.dtype, dev)
+                mem.ws = chunked.Workset(N, s.nparam, self.dtype, dev, M=s.M)
         if self._run is not None:
             self._run.destroy_graph()
         self._run, self._key = None, None
@@ -158,7 +159,7 @@ class BatchedLM:
             return None
         r = self._chunked(tol, mu, eta, acc, dec)
         r.set_iters(iters)
-        r.launch(chunked.chunk_schedule(int(iters), self.chunk))
+        r.launch(chunked.chunk_schedule(int(iters), self.chunk, self.max_chunks))
         return None
 
     def reset_state(self, stream=None):
@@ -177,7 +178,8 @@ class BatchedLM:
         replay_graph()."""
         r = self._chunked(tol, mu, eta, acc, dec)
         r.set_iters(iters)
-        r.capture(chunked.chunk_schedule(int(iters), self.chunk), init, fresh=True)
+        r.capture(chunked.chunk_schedule(int(iters), self.chunk, self.max_chunks), init,
+                  fresh=True)
         return r
 
     def replay_graph(self):

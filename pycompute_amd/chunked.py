@@ -17,30 +17,44 @@ from . import _lib
 from .lm import ptr
 
 
-def chunk_schedule(iters: int, k: int) -> list:
-    """Chunks of k iterations covering `iters` (the last one shorter)."""
+def chunk_schedule(iters: int, k: int, max_chunks: int = 0) -> list:
+    """Chunks of k iterations covering `iters` (the last one shorter). With
+    max_chunks > 0 the last of at most max_chunks chunks takes all remaining
+    iterations: the voxels still active after the first few chunks mostly run
+    to the iteration cap, so compacting them again only moves them."""
     if iters <= 0:
         return []
     k = max(1, int(k))
     out = [k] * (iters // k)
     if iters % k:
         out.append(iters % k)
+    m = int(max_chunks)
+    if m > 0 and len(out) > m:
+        out = out[:m - 1] + [sum(out[m - 1:])]
     return out
 
 
 class Workset:
     """Two packed queues (ping-pong) for one model's voxels. A queue entry is
     the state that changes (parameters, lambda, f, first f, step type,
-    iteration count) and the voxel's index; the kernel reads the voxel's data,
-    bounds and per-voxel consts from the caller's arrays through the index."""
+    iteration count), the voxel's index and, in the carried layout, its data,
+    bounds and per-voxel consts (ncm values per voxel), so that every later
+    chunk reads them coalesced. carried=False leaves those out: the kernel then
+    reads them from the caller's arrays through the index (less memory, more
+    scattered reads)."""
 
-    def __init__(self, N, n, dtype, device):
+    def __init__(self, N, n, dtype, device, M=0, ncm_per_voxel=0, carried=True):
         self.cap = max(int(N), 1)
         kw = dict(dtype=dtype, device=device)
         self.tensors = []
         self.structs = []
         for _ in range(2):
-            t = {"p": torch.empty((n, self.cap), **kw), "lam": torch.empty(self.cap, **kw),
+            t = {"y": torch.empty((M, self.cap), **kw) if carried else None,
+                 "cv": (torch.empty((ncm_per_voxel, self.cap), **kw)
+                        if carried and ncm_per_voxel else None),
+                 "lo": torch.empty((n, self.cap), **kw) if carried else None,
+                 "hi": torch.empty((n, self.cap), **kw) if carried else None,
+                 "p": torch.empty((n, self.cap), **kw), "lam": torch.empty(self.cap, **kw),
                  "f": torch.empty(self.cap, **kw), "f_first": torch.empty(self.cap, **kw),
                  "idx": torch.empty(self.cap, dtype=torch.int32, device=device),
                  "cnt": torch.empty(self.cap, dtype=torch.int32, device=device),

@@ -20,9 +20,9 @@
 // iterations over compacted work queues (one per model). After each chunk a
 // wavefront appends its still-active voxels to the next queue with one atomic
 // per wave; converged voxels write their results and leave (a queue entry
-// holds only the state that changes -- parameters, lambda, f, step type,
-// iteration count -- and the voxel index, through which the data and bounds
-// are read from the caller's arrays). Waves therefore
+// holds the state that changes -- parameters, lambda, f, step type, iteration
+// count -- and the voxel index; data and bounds travel with it or are read
+// from the caller's arrays through the index, see pca_ws). Waves therefore
 // stay full of active voxels instead of idling behind the slowest lane, and
 // several models (different n, M) share one persistent launch. Per-voxel
 // results are identical to pca_lm_fit.
@@ -356,6 +356,10 @@ pca_lm_fit(pca_lm_args a, pca_uconsts uc) {
 #ifndef PCA_NO_CHUNK
 
 struct pca_ws {          // packed queue of the voxels still iterating, entry e of cap:
+  real* y;               // (M, cap) measurements, or null
+  real* cv;              // (NC*M, cap) per-voxel consts, or null
+  real* lo;              // (NP, cap) or null
+  real* hi;              // (NP, cap) or null
   real* p;               // (NP, cap) parameters
   real* lam;             // (cap)
   real* f;               // (cap)
@@ -364,8 +368,12 @@ struct pca_ws {          // packed queue of the voxels still iterating, entry e 
   int* cnt;              // (cap) iterations executed in this run
   signed char* st;       // (cap)
   long long cap;
-};                       // data, bounds and per-voxel consts are read from the
-                         // caller's arrays through idx (they never change)
+};
+// Two queue layouts, chosen per call by y: carried (y, lo, hi and per-voxel
+// consts travel with the entry: coalesced reads, about 3.6x the queue memory
+// for IVIM) or compact (y null: they are read from the caller's arrays
+// through idx; compaction by per-wave atomics scrambles the queue order, so
+// later chunks gather from scattered voxels and fetch more bytes).
 
 struct pca_model_job {
   pca_lm_args a;         // the caller's arrays for this model
@@ -398,6 +406,7 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
   const pca_ws& out = J.ws[(A.chunk + 1) & 1];
   const Settings S = settings(a);
   const bool from_user = (A.chunk == 0);
+  const bool carried = in.y != nullptr;   // uniform: the queue layout of this call
 #ifdef PCA_CONSTS_PER_VOXEL
   constexpr bool per_voxel = true;
   static_assert(NC * M <= PCA_CV_REGS, "per-voxel consts too large for the chunked kernel");
@@ -427,23 +436,38 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
     } else {
       const long long cap = in.cap, N = a.N;
       v = in.idx[e];
+      if (carried) {
 #pragma unroll
-      for (int i = 0; i < M; ++i) V.y[i] = a.data[(long long)i * N + v];
+        for (int i = 0; i < M; ++i) V.y[i] = in.y[(long long)i * cap + e];
 #pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        V.p[j] = in.p[(long long)j * cap + e];
-        V.lo[j] = a.lb[(long long)j * N + v];
-        V.hi[j] = a.ub[(long long)j * N + v];
+        for (int j = 0; j < NP; ++j) {
+          V.lo[j] = in.lo[(long long)j * cap + e];
+          V.hi[j] = in.hi[(long long)j * cap + e];
+        }
+        if constexpr (per_voxel) {
+#pragma unroll
+          for (int q = 0; q < NCV; ++q) cv[q] = in.cv[(long long)q * cap + e];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < M; ++i) V.y[i] = a.data[(long long)i * N + v];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          V.lo[j] = a.lb[(long long)j * N + v];
+          V.hi[j] = a.ub[(long long)j * N + v];
+        }
+        if constexpr (per_voxel) {
+#pragma unroll
+          for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * N + v];
+        }
       }
+#pragma unroll
+      for (int j = 0; j < NP; ++j) V.p[j] = in.p[(long long)j * cap + e];
       V.lam = in.lam[e];
       V.f = in.f[e];
       V.f_first = in.f_first[e];
       V.st = in.st[e];
       cnt_total = in.cnt[e];
-      if constexpr (per_voxel) {
-#pragma unroll
-        for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * N + v];
-      }
     }
   }
   int cnt = 0;
@@ -473,6 +497,19 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
     const unsigned long long below = (lane_id() == 0) ? 0ull : (mask & (~0ull >> (64 - lane_id())));
     const long long pos = base + __popcll(below);
     const long long cap = out.cap;
+    if (carried) {
+#pragma unroll
+      for (int i = 0; i < M; ++i) out.y[(long long)i * cap + pos] = V.y[i];
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        out.lo[(long long)j * cap + pos] = V.lo[j];
+        out.hi[(long long)j * cap + pos] = V.hi[j];
+      }
+      if constexpr (per_voxel) {
+#pragma unroll
+        for (int q = 0; q < NCV; ++q) out.cv[(long long)q * cap + pos] = cv[q];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < NP; ++j) out.p[(long long)j * cap + pos] = V.p[j];
     out.lam[pos] = V.lam;

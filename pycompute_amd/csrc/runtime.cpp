@@ -81,7 +81,7 @@ extern "C" {
 
 const char* pca_last_error(void) { return g_err.c_str(); }
 
-int32_t pca_abi_version(void) { return 3; }
+int32_t pca_abi_version(void) { return 4; }
 
 int pca_compile(const char* source, const char* program_name, const char* const* options,
                 int32_t n_options, void** code, size_t* size) {
@@ -212,9 +212,20 @@ int pca_lm_chunked_launch(pca_kernel chunk_kernel, pca_kernel reset_kernel,
     const pca_model_job& j = jobs[m];
     if (j.a.N < 0 || j.a.N > 0x7fffffff) return fail(PCA_ERR_LIMIT, "voxels per model > 2^31-1");
     for (int q = 0; q < 2; ++q)
+    {
       if (j.a.N > 0 && (j.ws[q].cap < j.a.N || !j.ws[q].p || !j.ws[q].lam || !j.ws[q].f ||
                         !j.ws[q].f_first || !j.ws[q].idx || !j.ws[q].cnt || !j.ws[q].st))
         return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: working set smaller than N or NULL");
+      // a carried queue has y, lo and hi (cv for per-voxel consts); a compact
+      // one has none of them; both queues of a model use the same layout
+      const bool carried = j.ws[q].y != nullptr;
+      if (j.a.N > 0 && ((j.ws[q].lo != nullptr) != carried || (j.ws[q].hi != nullptr) != carried ||
+                        (j.ws[0].y != nullptr) != (j.ws[1].y != nullptr) ||
+                        (!carried && j.ws[q].cv != nullptr) ||
+                        (carried && j.a.consts && !j.uc && !j.ws[q].cv)))
+        return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: queue arrays y, lo, hi (and cv for "
+                                 "per-voxel consts) must be all set or all NULL");
+    }
   }
   for (int c = 0; c < n_chunks; ++c)
     if (chunk_iters[c] < 1) return fail(PCA_ERR_ARG, "pca_lm_chunked_launch: chunk of < 1 iteration");

@@ -67,6 +67,7 @@ size_t queue_bytes(int64_t N, int32_t n, size_t es) {
 // Carve one queue of capacity cap at base + off (device address arithmetic).
 pca_ws carve(unsigned char* base, size_t& off, int64_t cap, int32_t n, size_t es) {
   pca_ws w;
+  w.y = w.cv = w.lo = w.hi = nullptr;   // the compact layout
   auto take = [&](size_t bytes) {
     void* p = base + off;
     off += align_up(bytes);

@@ -141,7 +141,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
 
     def __init__(self, *args, block: int = lm.DEFAULT_BLOCK, fast_exp: bool = False,
                  keep_workspace: bool | None = None, strict: bool = False,
-                 compaction: bool = True, chunk: int = 8, unroll: int = 0,
+                 compaction: bool = True, chunk: int = 8, max_chunks: int = 0, unroll: int = 0,
                  waves_per_eu: int = 0, fast_div: bool = False, wide: bool | None = None,
                  tile: int = 136, reassoc: bool = False, sched_group: int = 0,
                  factored: bool = False, **kwargs):
@@ -190,6 +190,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         # unconverged voxels between chunks (identical results, fuller waves)
         self.compaction = bool(compaction)
         self.chunk = int(chunk)
+        self.max_chunks = int(max_chunks)
         self._ws = None
         self._uc_t = None
         self._chunk_state = None
@@ -362,7 +363,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         if self._use_chunked():
             r = self._chunked_run(tol, mu, eta, acc, dec)
             r.set_iters(iters)
-            r.launch(chunked.chunk_schedule(int(iters), self.chunk))
+            r.launch(chunked.chunk_schedule(int(iters), self.chunk, self.max_chunks))
             return None
         a = self.lm_args(iters, tol, mu, eta, acc, dec)
         k = self.program.kernel("pca_lm_fit")
@@ -383,7 +384,9 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             return self._chunk_state[1]
         dev = self.data_t.device
         if self._ws is None:
-            self._ws = chunked.Workset(self.Nprobs, self.nparam, self.dtype, dev)
+            ncm = self.nconst * self.ndata if self._cpv is not None else 0
+            self._ws = chunked.Workset(self.Nprobs, self.nparam, self.dtype, dev, M=self.ndata,
+                                       ncm_per_voxel=ncm)
             if self._uc is not None:
                 self._uc_t = torch.tensor(self._uc, dtype=self.dtype, device=dev)
         a = self.lm_args(0, tol, mu, eta, acc, dec)
@@ -409,7 +412,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             raise RuntimeError("graph capture needs the chunked path (compaction=True)")
         r = self._chunked_run(tol, mu, eta, acc, dec)
         r.set_iters(iters)
-        r.capture(chunked.chunk_schedule(int(iters), self.chunk), init, fresh=True)
+        r.capture(chunked.chunk_schedule(int(iters), self.chunk, self.max_chunks), init,
+                  fresh=True)
         return r
 
     def replay_graph(self):

@@ -188,3 +188,18 @@ def test_autotune_pick_ties_and_agreement():
     assert autotune.pick(r)["knobs"] == {"k": 1}
     r[2].update(worse=0.0025, better=0.001)             # within 2x + 0.1%
     assert autotune.pick(r)["knobs"] == {"k": 2}
+
+
+def test_chunk_schedule():
+    from pycompute_amd.chunked import chunk_schedule
+    assert chunk_schedule(100, 16) == [16] * 6 + [4]
+    assert chunk_schedule(100, 16, 4) == [16, 16, 16, 52]
+    assert chunk_schedule(30, 16, 4) == [16, 14]
+    assert chunk_schedule(6, 16, 4) == [6]
+    assert chunk_schedule(0, 16, 4) == []
+    for it in range(1, 120):
+        for k in (1, 5, 16):
+            for m in (0, 1, 2, 4):
+                s = chunk_schedule(it, k, m)
+                assert sum(s) == it and all(x >= 1 for x in s)
+                assert m == 0 or len(s) <= m
