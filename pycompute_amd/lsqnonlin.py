@@ -198,11 +198,12 @@ class SecondOrderLevenbergMarquardt(HipFunction):
                            fast_div=bool(fast_div), reassoc=bool(reassoc),
                            sched_group=int(sched_group), factored=bool(factored))
         self.keep_workspace = (self.nparam <= 8) if keep_workspace is None else bool(keep_workspace)
-        # wide: the kernel for models with more parameters than one lane's
-        # registers hold (csrc/lm_glm.hip; generalised linear models with
-        # voxel-uniform consts, n > 8). None = use it whenever it applies.
+        # wide: the kernels for models with more parameters than one lane's
+        # registers hold (n > 8): csrc/lm_glm.hip for generalised linear
+        # models with voxel-uniform consts, csrc/lm_wide.hip (Hessian kept in
+        # LDS) for any other model. None = use them whenever they apply.
         self.wide, self.tile = wide, int(tile)
-        self._glm, self._glm_tab = None, None
+        self._glm, self._glm_tab, self._wide_prog = None, None, None
         # more consts than the kernel-argument block holds: per-voxel layout
         self._uc_fits = self.nconst * self.ndata <= _lib.PCA_UC_MAX
         if self.expr_true:
@@ -211,10 +212,10 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             if wide is not False:
                 from . import wide as _wide
                 self._glm = _wide.glm_program(self._spec_uniform, self.tile)
-                if wide and self._glm is None:
-                    raise ValueError("the wide kernel needs n > 8, a non-strict build, a "
-                                     "model phi(eta) with eta affine in the parameters and a "
-                                     "tile whose parked Hessian entries fit in 64 KB of LDS")
+                if wide and self._glm is None and _wide.wide_program(
+                        self._spec(not self._uc_fits), self.tile) is None:
+                    raise ValueError("the wide kernels need n > 8 and a packed Hessian that "
+                                     "fits in the 160 KB of LDS of a workgroup")
         else:
             self.funcid = self._plugin.funcid
         self.Nprobs = 1
@@ -308,6 +309,11 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         self._glm_tab = None
         if self._glm is not None and self._uc is not None:
             self._glm_tab = self._glm.table(self._uc, self.data_t.device)
+        self._wide_prog = None
+        if (self.expr_true and self._glm_tab is None and self.wide is not False
+                and self.nparam > 8):
+            from . import wide as _wide
+            self._wide_prog = _wide.wide_program(self._spec(self._cpv is not None), self.tile)
         dev = self.data_t.device
         kw = dict(dtype=self.dtype, device=dev)
         self.lam_t = torch.full((1, N), 5.0, **kw)
@@ -359,6 +365,11 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             return self._plugin.run(self, iters, tol, mu, eta, acc, dec)
         if self._glm_tab is not None:
             self._glm.launch(self.lm_args(iters, tol, mu, eta, acc, dec), self._glm_tab)
+            return None
+        if self._wide_prog is not None:
+            a = self.lm_args(iters, tol, mu, eta, acc, dec)
+            a.h_out = a.hl_out = None           # the wide kernels write g and step only
+            self._wide_prog.launch(a, self._ucbuf, self._ucbytes)
             return None
         if self._use_chunked():
             r = self._chunked_run(tol, mu, eta, acc, dec)

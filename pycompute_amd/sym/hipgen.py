@@ -448,7 +448,10 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
            f"  static constexpr int NP = {n};",
            f"  static constexpr int NC = {dag.nconst};",
            f"  static constexpr int NH = {nh};",
-           f"  static constexpr bool HLD_DAMP_ONLY = {'true' if reassoc else 'false'};"]
+           f"  static constexpr bool HLD_DAMP_ONLY = {'true' if reassoc else 'false'};",
+           # structurally non-zero second derivatives (packed order)
+           "  static constexpr bool HNZ[NH] = {"
+           + ", ".join("true" if z else "false" for z in dag.hes_nonzero()) + "};"]
 
     # resid: model - y (the F kernel body, lsqnonlin.py:60-66)
     pr = _Printer(dag, dtype)

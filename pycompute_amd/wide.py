@@ -1,5 +1,7 @@
-"""Programs of the wide-model kernel (csrc/lm_glm.hip) for generalised linear
-models with more parameters than one lane's registers hold (n > 8)."""
+"""Programs of the wide-model kernels for models with more parameters than one
+lane's registers hold (n > 8): csrc/lm_glm.hip for generalised linear models
+(voxel-uniform consts), csrc/lm_wide.hip for any other generated model (the
+Hessian tiles accumulated in registers and kept in LDS)."""
 from __future__ import annotations
 
 import ctypes
@@ -14,7 +16,7 @@ from .sym import glm, hipgen
 
 WIDE_MIN_PARAMS = 9
 DEFAULT_TILE = 136
-LDS_LIMIT = 65536   # bytes of LDS one workgroup may allocate
+LDS_LIMIT = 160 * 1024   # bytes of LDS one gfx950 workgroup may allocate
 
 
 def tile_rows(n: int, tile_max: int) -> list:
@@ -113,9 +115,85 @@ def glm_flops(n: int, M: int, generic: dict) -> dict:
             "F_solve": generic["F_solve"], "F_step": generic["F_step"]}
 
 
+class WideProgram:
+    """The general wide kernel (pca_lm_wide) for a generated model with n > 8
+    parameters: the model struct of the generated module, the packed Hessian
+    in LDS (n(n+1)/2 columns of 64 lanes), tiles of at most tile_max entries
+    accumulated in registers."""
+
+    def __init__(self, spec: lm.LMSpec, tile_max: int = DEFAULT_TILE):
+        self.spec = spec
+        self.dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
+        self.tile_max = int(tile_max)
+        n = spec.nparam
+        self.lds_bytes = wide_lds_bytes(n, spec.dtype)
+        if self.lds_bytes > LDS_LIMIT:
+            raise ValueError(f"{n} parameters keep {self.lds_bytes} bytes of Hessian in LDS per "
+                             f"workgroup (limit {LDS_LIMIT})")
+        self.funcid = spec.funcid() + f"_wide{self.tile_max}" + ("_strict" if spec.strict else "")
+        head = [f"// generated: {self.funcid}", f"// model: {spec.expr}",
+                f"#define PCA_M {spec.M}", f"#define PCA_UC_MAX {_lib.PCA_UC_MAX}",
+                f"#define PCA_TILE_MAX {self.tile_max}"]
+        if spec.dtype == "float64":
+            head.append("#define PCA_REAL_DOUBLE 1")
+        if spec.consts_per_voxel:
+            head.append("#define PCA_CONSTS_PER_VOXEL 1")
+        if spec.fast_exp:
+            head.append("#define PCA_FAST_EXP 1")
+        if spec.strict:
+            head.append("#define PCA_STRICT 1")
+        if spec.reassoc:
+            head.append("#define PCA_FAST_SOLVE 1")
+        model = hipgen.emit_model(self.dag, "Model", spec.dtype, reassoc=spec.reassoc)
+        self.source = "\n".join(["\n".join(head), read_csrc("strict_math.h"),
+                                 read_csrc("prelude.hip"), model, read_csrc("lm_solver.hip"),
+                                 read_csrc("lm_wide.hip")])
+        options = list(BASE_OPTIONS)
+        if spec.strict:
+            options.append("-ffp-contract=off")
+        if spec.fast_div:
+            options = [o for o in options if o != "-fhip-fp32-correctly-rounded-divide-sqrt"]
+            options.append("-fno-hip-fp32-correctly-rounded-divide-sqrt")
+        self.module = Module(self.source, self.funcid, options)
+        self.flops = hipgen.flops_per_iteration(self.dag, spec.M)
+
+    def launch(self, args: _lib.LMArgs, ucbuf, ucbytes, stream=None):
+        s = lm.stream_handle() if stream is None else stream
+        _lib.check(_lib.lib().pca_lm_launch(self.module.kernel("pca_lm_wide"), ctypes.byref(args),
+                                            ucbuf, ucbytes, 64, s), "pca_lm_launch (wide)")
+
+
+def wide_lds_bytes(n: int, dtype: str) -> int:
+    """LDS of the general wide kernel: the packed Hessian, one column per lane."""
+    return n * (n + 1) // 2 * 64 * (8 if dtype == "float64" else 4)
+
+
 _programs: dict = {}
 _lock = threading.Lock()
 _key_locks: dict = {}
+_wide_programs: dict = {}
+
+
+def wide_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> WideProgram | None:
+    """The general wide-kernel program for ``spec`` (any generated model with
+    n > 8 whose Hessian fits in LDS), or None."""
+    if spec.nparam < WIDE_MIN_PARAMS or wide_lds_bytes(spec.nparam, spec.dtype) > LDS_LIMIT:
+        return None
+    if not spec.consts_per_voxel and spec.nconst * spec.M > _lib.PCA_UC_MAX:
+        return None
+    key = (spec, tile_max)
+    with _lock:
+        if key in _wide_programs:
+            return _wide_programs[key]
+        klock = _key_locks.setdefault(("wide",) + key, threading.Lock())
+    with klock:
+        with _lock:
+            if key in _wide_programs:
+                return _wide_programs[key]
+        prog = WideProgram(spec, tile_max)
+        with _lock:
+            _wide_programs[key] = prog
+        return prog
 
 
 def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> GLMProgram | None:

@@ -14,7 +14,9 @@ same inputs by the FP32 and FP64 oracles:
     oracle's own rounding error, on >= 99.9% of voxels (where the guess
     fits well, residuals are small against the signal and FP32 rounding of
     the model alone moves the SSR by more than 1e-4);
-  * final SSR within 1e-4 of the FP32 oracle on >= 98% of voxels;
+  * final SSR within 1e-4 of the FP32 oracle on >= 98% of voxels (for an
+    ill-conditioned model where the FP32 oracle itself agrees with the FP64
+    one on fewer, on that fraction less two points);
   * parameters within 1e-4 of the FP64 optimum on at least as many voxels,
     less two points, as the FP32 oracle's own parameters are;
   * residual tail: for t in (1e-4, 1e-3, 1e-2, 1e-1), the fraction of voxels
@@ -48,13 +50,34 @@ def oracle_pair(model, consts, y, lb, ub, p0, iters):
     return r32, r64
 
 
-def check(tag, model, consts, y, gpu_params, r32, r64, gpu_first_f=None, min_agree=0.98):
-    """Assert the bars above; returns the measured figures."""
+def gen_oracle_pair(expr, pars, consts_names, consts, y, lb, ub, p0, iters):
+    """oracle_pair for a generated model: the oracle's LM driver over the
+    model compiled for the host (oracle/genmodel.py), FP32 and FP64."""
+    import oracle.genmodel as gm
+    y = np.ascontiguousarray(y)
+    d = np.float64
+    r32 = gm.lm_run(expr, pars, consts_names, consts, y, lb, ub, p0, iters, dtype=np.float32)
+    r64 = gm.lm_run(expr, pars, consts_names, consts, y.astype(d), lb.astype(d), ub.astype(d),
+                    p0.astype(d), iters, dtype=d)
+    ssr = lambda p: gm.ssr(expr, pars, consts_names, consts, y.astype(d), p, dtype=d)  # noqa
+    r32["S"], r64["S"] = ssr(r32["params"].astype(d)), ssr(r64["params"])
+    return r32, r64, ssr
+
+
+def check(tag, model, consts, y, gpu_params, r32, r64, gpu_first_f=None, min_agree=0.98,
+          ssr_fn=None):
+    """Assert the bars above; returns the measured figures. ssr_fn(params):
+    the float64 SSR for a generated model (default: the oracle's own model)."""
     d = np.float64
     pg = np.asarray(gpu_params, d)
-    Sg = oracle.ssr(model, consts, np.ascontiguousarray(y).astype(d), pg, dtype=d)
+    Sg = ssr_fn(pg) if ssr_fn else oracle.ssr(model, consts, np.ascontiguousarray(y).astype(d),
+                                              pg, dtype=d)
     S32, S64 = r32["S"], r64["S"]
     agree = float(np.mean(np.abs(Sg - S32) <= 1e-4 * np.abs(S32)))
+    # for an ill-conditioned model FP32 itself strays: the bar is what the FP32
+    # oracle reaches against the FP64 one, less two points, when that is lower
+    agree_32 = float(np.mean(np.abs(S32 - S64) <= 1e-4 * np.abs(S64)))
+    bar = min(min_agree, agree_32 - 0.02)
 
     def prel(a, b):
         return (np.abs(a - b) / np.maximum(np.abs(b), 1e-7)).max(axis=0)
@@ -79,7 +102,7 @@ def check(tag, model, consts, y, gpu_params, r32, r64, gpu_first_f=None, min_agr
         ok = (eg0 <= 1e-4 * np.abs(f64)) | (eg0 <= 3 * e320)
         out["first_f_ok"] = float(np.mean(ok))
         assert out["first_f_ok"] >= 0.999, (tag, "first_f", out["first_f_ok"])
-    assert agree >= min_agree, (tag, "final SSR vs FP32 oracle", agree)
+    assert agree >= bar, (tag, "final SSR vs FP32 oracle", agree, bar)
     assert par_g >= par_32 - 0.02, (tag, "parameters vs FP64", par_g, par_32)
     slack = 3e-4 + 2.0 / eg.size           # two voxels of a small sample
     for t in TAILS:

@@ -67,13 +67,18 @@ def test_tile_rows_partition_the_packed_triangle(n, tile):
 
 
 def test_wide_kernel_lds_budget():
-    """The LDS park holds every tile but the last, one column per lane: the
-    22-parameter tiles in use fit one workgroup's 64 KB in FP32; FP64 at tile
-    136 does not, so the wide kernel declines (the register kernel runs)."""
-    from pycompute_amd import lm, wide
+    """The GLM kernel's LDS park holds every tile but the last, one column per
+    lane; the general wide kernel keeps the whole packed Hessian. Both fit
+    the 160 KB a gfx950 workgroup may allocate for the 22-parameter model in
+    FP32 and FP64; past the limit the programs decline."""
+    from pycompute_amd import wide
     for tile in (40, 100, 136, 170):
         assert wide.lds_bytes(22, tile, "float32") <= wide.LDS_LIMIT
+        assert wide.lds_bytes(22, tile, "float64") <= wide.LDS_LIMIT
+    assert wide.LDS_LIMIT == 160 * 1024
     assert wide.lds_bytes(22, 136, "float32") == 136 * 64 * 4
-    assert wide.lds_bytes(22, 136, "float64") > wide.LDS_LIMIT
-    spec = lm.LMSpec(DKI.expr, DKI.pars, DKI.consts, 61, "float64")
-    assert wide.glm_program(spec, 136) is None
+    assert wide.wide_lds_bytes(22, "float32") == 253 * 64 * 4
+    assert wide.wide_lds_bytes(22, "float64") == 253 * 64 * 8
+    # 36 parameters: 666 entries, 170 KB in FP32, over the limit
+    assert wide.wide_lds_bytes(36, "float32") > wide.LDS_LIMIT
+    assert wide.lds_bytes(60, 40, "float32") > wide.LDS_LIMIT
