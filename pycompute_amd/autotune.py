@@ -16,6 +16,9 @@ of the caller's own problem:
   * factored:      parameter-only factors of the Jacobian and second
                    derivatives applied once per iteration (with reassoc)
   * sched_group:   register fence every k measurements (bounds registers)
+  * vpl:           voxels per lane (1 or 2; 64 * vpl voxels per wavefront):
+                   two voxels in lockstep interleave two independent chains
+                   at twice the registers per lane
   * waves_per_eu:  occupancy hint (mixed-model batches, batch_candidates())
   * tile:          wide-model kernel register/LDS split (wide_candidates())
 
@@ -36,7 +39,7 @@ from . import program
 
 DEFAULT = {"block": 256, "chunk": 16, "max_chunks": 0, "unroll": 0, "fast_exp": False,
            "fast_div": False, "reassoc": False, "waves_per_eu": 0, "factored": False,
-           "sched_group": 0}
+           "sched_group": 0, "vpl": 1}
 
 
 def _cache_path() -> str:
@@ -81,7 +84,14 @@ def candidates(nparam: int, allow_fast: bool):
             for block in (256, 128):
                 out.append({"block": block, "chunk": chunk, "max_chunks": mc, "unroll": 0,
                             "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                            "waves_per_eu": wpe, "factored": fac, "sched_group": sg})
+                            "waves_per_eu": wpe, "factored": fac, "sched_group": sg,
+                            "vpl": 1})
+        if sg == 0 and wpe == 0 and nparam <= 8:
+            # two voxels per lane (one wavefront per SIMD for most models)
+            for chunk, mc in ((16, 0), (16, 4)):
+                out.append({"block": 128, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                            "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2})
     return out
 
 
@@ -98,7 +108,11 @@ def batch_candidates(allow_fast: bool = True):
             for wpe in (0, 2):
                 out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
                             "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                            "waves_per_eu": wpe, "factored": fac, "sched_group": 0})
+                            "waves_per_eu": wpe, "factored": fac, "sched_group": 0,
+                            "vpl": 1})
+        out.append({"block": 128, "chunk": 16, "max_chunks": 4, "unroll": 0,
+                    "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                    "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2})
     return out
 
 

@@ -68,14 +68,14 @@ class BatchedLM:
     """models: list of (model, ndata), model an Expr, a models.ModelSpec or an
     (expr, pars, consts) triple; at most 4 models. Consts must be
     voxel-uniform (one protocol per model). Knobs as SecondOrderLevenbergMarquardt's
-    chunked path: block, chunk, max_chunks, unroll, fast_exp, fast_div, strict."""
+    chunked path: block, chunk, max_chunks, vpl, unroll, fast_exp, fast_div, strict."""
 
     def __init__(self, models, dtype=torch.float32, block: int = lm.DEFAULT_BLOCK,
                  chunk: int = 8, max_chunks: int = 0, fast_exp: bool = False,
                  fast_div: bool = False,
                  strict: bool = False, unroll: int = 0, waves_per_eu: int = 0,
                  reassoc: bool = False, sched_group: int = 0, factored: bool = False,
-                 compaction: bool = True, **_):
+                 vpl: int = 1, compaction: bool = True, **_):
         if not 1 <= len(models) <= 4:
             raise ValueError("a batch holds 1 to 4 models")
         self.dtype_name = lm.dtype_name(dtype)
@@ -89,7 +89,7 @@ class BatchedLM:
                                    bool(strict), unroll=int(unroll),
                                    waves_per_eu=int(waves_per_eu), fast_div=bool(fast_div),
                                    reassoc=bool(reassoc), sched_group=int(sched_group),
-                                   factored=bool(factored)))
+                                   factored=bool(factored), vpl=int(vpl)))
         self.members = [Member(s) for s in specs]
         self.program = lm.get_program(specs[0], tuple(specs[1:]))
         self.funcid = self.program.funcid

@@ -75,7 +75,9 @@ class ChunkedRun:
         self.jobs = jobs
         self.block = block
         n_total = sum(int(a.N) for _, a, _, _ in jobs)
-        waves = (n_total + 63) // 64
+        # one wavefront task covers 64 * vpl queue slots
+        slots = 64 * int(getattr(program.spec, "vpl", 1))
+        waves = (n_total + slots - 1) // slots
         self.grid = grid or max(1, min((waves * 64 + block - 1) // block, 8192))
         arr = (_lib.ModelJob * len(jobs))()
         for i, (_, a, uc, ws) in enumerate(jobs):

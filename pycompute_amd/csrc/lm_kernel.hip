@@ -54,6 +54,12 @@
 #ifndef PCA_FACTORED
 #define PCA_FACTORED 0
 #endif
+// Voxels per lane of the chunked kernel (1 or 2): a lane iterates PCA_VPL
+// queue entries in lockstep, so their independent dependency chains
+// interleave (64 * PCA_VPL voxels per wavefront).
+#ifndef PCA_VPL
+#define PCA_VPL 1
+#endif
 namespace pca {
 __device__ __forceinline__ void reg_fence(real& x) { asm volatile("" : "+v"(x)); }
 template <int K>
@@ -249,6 +255,179 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
   }
 }
 
+// The iteration of iterate() for the VPL voxels of one lane together
+// (VPL > 1). Every step is written for all of them at once, so the compiler
+// interleaves their independent chains. There is no early return: a voxel that
+// converges in this iteration, or had already stopped (st == 0), runs the
+// rest of the iteration too and keeps its state through selects. Its f is
+// kept as it was when it had already stopped; g, h, hld and s are recomputed,
+// and since a converged voxel's p and lambda never change afterwards they come
+// out equal to its converging iteration's (they are only stored for a voxel
+// that iterated in this launch). cf(u, k, i): const k of measurement i for
+// voxel u. Per-voxel results equal iterate()'s.
+template <class Mdl, int M, int UNROLL, int VPL, class CF>
+__device__ __forceinline__ void iterate_n(Voxel<Mdl, M> (&V)[VPL], const Settings& S,
+                                          const CF& cf) {
+  constexpr int NP = Mdl::NP, NC = Mdl::NC, NH = Mdl::NH;
+  constexpr bool factored = PCA_FACTORED && Mdl::FACTORED;
+  bool active[VPL];
+  real f[VPL];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    active[u] = V[u].st != 0;
+    f[u] = 0;
+  }
+  // fghhl (lsqnonlin.py:206-346)
+  if constexpr (factored) {
+    typename Mdl::Pre P[VPL];
+    real s[VPL][Mdl::NS], q[VPL][NH], dmp[VPL][NP];
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) {
+      Mdl::prep(V[u].p, V[u].lam, P[u]);
+#pragma unroll
+      for (int k = 0; k < Mdl::NS; ++k) s[u][k] = 0;
+#pragma unroll
+      for (int l = 0; l < NH; ++l) q[u][l] = 0;
+#pragma unroll
+      for (int j = 0; j < NP; ++j) dmp[u][j] = 0;
+    }
+#pragma unroll UNROLL
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+      for (int u = 0; u < VPL; ++u) {
+        real c[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) c[k] = cf(u, k, i);
+        Mdl::accf(P[u], c, V[u].y[i], f[u], s[u], q[u], dmp[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < VPL; ++u)
+      Mdl::finish(V[u].p, P[u].L, s[u], q[u], dmp[u], V[u].g, V[u].h, V[u].hld);
+  } else {
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        V[u].g[j] = 0;
+        V[u].hld[j] = 0;
+      }
+#pragma unroll
+      for (int l = 0; l < NH; ++l) V[u].h[l] = 0;
+    }
+#pragma unroll UNROLL
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+      for (int u = 0; u < VPL; ++u) {
+        real c[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) c[k] = cf(u, k, i);
+        Mdl::accum(V[u].p, c, V[u].y[i], V[u].lam, f[u], V[u].g, V[u].h, V[u].hld);
+      }
+    }
+  }
+  bool go[VPL];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    if constexpr (Mdl::HLD_DAMP_ONLY) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) V[u].hld[j] += V[u].h[tri(j, j)];
+    }
+    // step and NaN/Inf scrub (lsqnonlin.py:843-891)
+    gmw81_solve<NP>(V[u].h, V[u].hld, V[u].g, V[u].s);
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      if (pca_nonfinite(V[u].s[j])) V[u].s[j] = 0;
+    // projected-gradient convergence (lsqnonlin.py:691-729)
+    bool clamped = false;
+    real norm = 0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      real t = V[u].p[j] - V[u].g[j];
+      const bool above = t > V[u].hi[j], below = t < V[u].lo[j];
+      clamped = clamped | above | below;
+      t = above ? V[u].hi[j] : (below ? V[u].lo[j] : t);
+      t = V[u].p[j] - t;
+      norm += t * t;
+    }
+    const bool conv = norm < S.tol * ((real)1 + f[u]) && ((V[u].st & 1) || clamped);
+    go[u] = active[u] && !conv;
+    V[u].f = active[u] ? f[u] : V[u].f;
+  }
+
+  // trial points and their SSR (unclamped, lsqnonlin.py:902-911)
+  real ptp[VPL][NP], ftp[VPL];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    ftp[u] = 0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) ptp[u][j] = V[u].p[j] - V[u].s[j];
+  }
+  if constexpr (factored) {
+    typename Mdl::RPre R[VPL];
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) Mdl::rprep(ptp[u], R[u]);
+#pragma unroll UNROLL
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+      for (int u = 0; u < VPL; ++u) {
+        real c[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) c[k] = cf(u, k, i);
+        const real r = Mdl::residf(R[u], c, V[u].y[i]);
+        ftp[u] += r * r;
+      }
+    }
+  } else {
+#pragma unroll UNROLL
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+      for (int u = 0; u < VPL; ++u) {
+        real c[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) c[k] = cf(u, k, i);
+        const real r = Mdl::resid(ptp[u], c, V[u].y[i]);
+        ftp[u] += r * r;
+      }
+    }
+  }
+
+  // gain ratio, step type and clamp (lsqnonlin.py:432-498, 584-607), kept
+  // only by voxels that iterate on
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    const real actual = (real)0.5 * (V[u].f - ftp[u]);
+    real pred = 0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        const real e = V[u].h[tri(i, j)] * V[u].s[i] * V[u].s[j];
+        if (i == j)
+          pred -= e;
+        else
+          pred -= (real)2 * e;
+      }
+    pred *= (real)0.5;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) pred += V[u].s[i] * V[u].g[i];
+    const real rho = actual / pred;
+    const bool accept = rho > S.mu && actual > 0;
+    const bool good = accept && rho > S.eta;
+    const bool nondesc = pred < 0;
+    real lam = good ? V[u].lam * S.acc : (accept ? V[u].lam : V[u].lam * S.dec);
+    lam = nondesc ? lam * S.dec : lam;
+    V[u].lam = go[u] ? lam : V[u].lam;
+    V[u].st = go[u] ? ((good ? 1 : (accept ? 2 : 4)) | (nondesc ? 8 : 0)) : 0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const real q = accept ? ptp[u][j] : V[u].p[j];
+      const real c = q > V[u].hi[j] ? V[u].hi[j] : (q < V[u].lo[j] ? V[u].lo[j] : q);
+      V[u].p[j] = go[u] ? c : V[u].p[j];
+    }
+  }
+}
+
 // Write a voxel's results to the caller's arrays (original index v).
 template <class Mdl, int M>
 __device__ __forceinline__ void store_result(const pca_lm_args& a, long long v,
@@ -398,8 +577,8 @@ namespace pca {
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 template <int MI, class Mdl, int M, int UNROLL>
-__device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e, bool valid) {
-  constexpr int NP = Mdl::NP, NC = Mdl::NC;
+__device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e0, long long count) {
+  constexpr int NP = Mdl::NP, NC = Mdl::NC, VPL = PCA_VPL;
   const pca_model_job& J = A.job[MI];
   const pca_lm_args& a = J.a;
   const pca_ws& in = J.ws[A.chunk & 1];
@@ -414,112 +593,165 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e,
   constexpr bool per_voxel = false;
 #endif
 
-  Voxel<Mdl, M> V;
-  long long v = 0;
-  int cnt_total = 0;
+  // slot u of this lane: queue entry e0 + 64 u (each load of a wavefront
+  // stays one contiguous row segment)
+  Voxel<Mdl, M> V[VPL];
+  long long v[VPL];
+  bool valid[VPL];
+  int cnt_total[VPL];
   constexpr int NCV = (NC * M <= PCA_CV_REGS) ? NC * M : 1;
-  real cv[NCV];
-  if (valid) {
+  real cv[VPL][NCV];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    const long long e = e0 + 64 * u;
+    valid[u] = e < count;
+    v[u] = 0;
+    cnt_total[u] = 0;
+    if constexpr (VPL > 1) {
+      if (!valid[u]) {   // an empty slot computes on zeros and is never stored
+#pragma unroll
+        for (int i = 0; i < M; ++i) V[u].y[i] = 0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) V[u].p[j] = V[u].lo[j] = V[u].hi[j] = 0;
+#pragma unroll
+        for (int q = 0; q < NCV; ++q) cv[u][q] = 0;
+        V[u].lam = V[u].f = V[u].f_first = 0;
+        V[u].st = 0;
+      }
+    }
+    if (!valid[u]) continue;
     if (from_user) {
-      v = e;
-      load_user(a, v, V);
+      v[u] = e;
+      load_user(a, e, V[u]);
       if (A.fresh) {
-        V.lam = (real)5;
-        V.st = 1;
-        V.f = V.f_first = (real)0;
-        if (a.n_iter) a.n_iter[v] = 0;
+        V[u].lam = (real)5;
+        V[u].st = 1;
+        V[u].f = V[u].f_first = (real)0;
+        if (a.n_iter) a.n_iter[e] = 0;
       }
       if constexpr (per_voxel) {
 #pragma unroll
-        for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * a.N + v];
+        for (int q = 0; q < NCV; ++q) cv[u][q] = a.consts[(long long)q * a.N + e];
       }
     } else {
       const long long cap = in.cap, N = a.N;
-      v = in.idx[e];
+      const long long vv = in.idx[e];
+      v[u] = vv;
       if (carried) {
 #pragma unroll
-        for (int i = 0; i < M; ++i) V.y[i] = in.y[(long long)i * cap + e];
+        for (int i = 0; i < M; ++i) V[u].y[i] = in.y[(long long)i * cap + e];
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
-          V.lo[j] = in.lo[(long long)j * cap + e];
-          V.hi[j] = in.hi[(long long)j * cap + e];
+          V[u].lo[j] = in.lo[(long long)j * cap + e];
+          V[u].hi[j] = in.hi[(long long)j * cap + e];
         }
         if constexpr (per_voxel) {
 #pragma unroll
-          for (int q = 0; q < NCV; ++q) cv[q] = in.cv[(long long)q * cap + e];
+          for (int q = 0; q < NCV; ++q) cv[u][q] = in.cv[(long long)q * cap + e];
         }
       } else {
 #pragma unroll
-        for (int i = 0; i < M; ++i) V.y[i] = a.data[(long long)i * N + v];
+        for (int i = 0; i < M; ++i) V[u].y[i] = a.data[(long long)i * N + vv];
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
-          V.lo[j] = a.lb[(long long)j * N + v];
-          V.hi[j] = a.ub[(long long)j * N + v];
+          V[u].lo[j] = a.lb[(long long)j * N + vv];
+          V[u].hi[j] = a.ub[(long long)j * N + vv];
         }
         if constexpr (per_voxel) {
 #pragma unroll
-          for (int q = 0; q < NCV; ++q) cv[q] = a.consts[(long long)q * N + v];
+          for (int q = 0; q < NCV; ++q) cv[u][q] = a.consts[(long long)q * N + vv];
         }
       }
 #pragma unroll
-      for (int j = 0; j < NP; ++j) V.p[j] = in.p[(long long)j * cap + e];
-      V.lam = in.lam[e];
-      V.f = in.f[e];
-      V.f_first = in.f_first[e];
-      V.st = in.st[e];
-      cnt_total = in.cnt[e];
+      for (int j = 0; j < NP; ++j) V[u].p[j] = in.p[(long long)j * cap + e];
+      V[u].lam = in.lam[e];
+      V[u].f = in.f[e];
+      V[u].f_first = in.f_first[e];
+      V[u].st = in.st[e];
+      cnt_total[u] = in.cnt[e];
     }
   }
-  int cnt = 0;
-  if (valid) {
+  int cnt[VPL];
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) cnt[u] = 0;
 #ifdef PCA_CONSTS_PER_VOXEL
-    auto cf = [&](int k, int i) { return cv[(k * M + i) % NCV]; };
+  auto cfu = [&](int u, int k, int i) { return cv[u][(k * M + i) % NCV]; };
 #else
-    const real* __restrict__ uc = J.uc;
-    auto cf = [&](int k, int i) { return uc[k * M + i]; };
+  const real* __restrict__ uc = J.uc;
+  auto cfu = [&](int u, int k, int i) { return uc[k * M + i]; };
 #endif
-    for (int j = 0; j < A.k && V.st != 0; ++j) {
-      iterate<Mdl, M, UNROLL>(V, S, cf);
-      if (A.it0 + j == 0) V.f_first = V.f;
-      ++cnt;
+  if constexpr (VPL == 1) {
+    if (valid[0]) {
+      auto cf = [&](int k, int i) { return cfu(0, k, i); };
+      for (int j = 0; j < A.k && V[0].st != 0; ++j) {
+        iterate<Mdl, M, UNROLL>(V[0], S, cf);
+        if (A.it0 + j == 0) V[0].f_first = V[0].f;
+        ++cnt[0];
+      }
+    }
+  } else {
+    for (int j = 0; j < A.k; ++j) {
+      bool act[VPL], any = false;
+#pragma unroll
+      for (int u = 0; u < VPL; ++u) {
+        act[u] = V[u].st != 0;   // empty slots hold st = 0
+        any = any || act[u];
+      }
+      if (!any) break;
+      iterate_n<Mdl, M, UNROLL, VPL>(V, S, cfu);
+#pragma unroll
+      for (int u = 0; u < VPL; ++u) {
+        if (A.it0 + j == 0) V[u].f_first = V[u].f;
+        cnt[u] += act[u] ? 1 : 0;
+      }
     }
   }
-  cnt_total += cnt;
-  const bool keep = valid && V.st != 0 && !A.last;
   // one atomic per wavefront reserves the slots of its surviving voxels
-  const unsigned long long mask = __ballot(keep);
-  const int n_keep = __popcll(mask);
+  bool keep[VPL];
+  unsigned long long mask[VPL];
+  int n_keep = 0;
+#pragma unroll
+  for (int u = 0; u < VPL; ++u) {
+    cnt_total[u] += cnt[u];
+    keep[u] = valid[u] && V[u].st != 0 && !A.last;
+    mask[u] = __ballot(keep[u]);
+    n_keep += __popcll(mask[u]);
+  }
   int base = 0;
   if (lane_id() == 0 && n_keep > 0)
     base = atomicAdd(&A.counts[(A.chunk + 1) * PCA_NMODELS + MI], n_keep);
   base = __shfl(base, 0);
-  if (keep) {
-    const unsigned long long below = (lane_id() == 0) ? 0ull : (mask & (~0ull >> (64 - lane_id())));
-    const long long pos = base + __popcll(below);
-    const long long cap = out.cap;
-    if (carried) {
+  const unsigned long long below_lane = (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id()));
 #pragma unroll
-      for (int i = 0; i < M; ++i) out.y[(long long)i * cap + pos] = V.y[i];
+  for (int u = 0; u < VPL; ++u) {
+    if (keep[u]) {
+      const long long pos = base + __popcll(mask[u] & below_lane);
+      const long long cap = out.cap;
+      if (carried) {
 #pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        out.lo[(long long)j * cap + pos] = V.lo[j];
-        out.hi[(long long)j * cap + pos] = V.hi[j];
+        for (int i = 0; i < M; ++i) out.y[(long long)i * cap + pos] = V[u].y[i];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          out.lo[(long long)j * cap + pos] = V[u].lo[j];
+          out.hi[(long long)j * cap + pos] = V[u].hi[j];
+        }
+        if constexpr (per_voxel) {
+#pragma unroll
+          for (int q = 0; q < NCV; ++q) out.cv[(long long)q * cap + pos] = cv[u][q];
+        }
       }
-      if constexpr (per_voxel) {
 #pragma unroll
-        for (int q = 0; q < NCV; ++q) out.cv[(long long)q * cap + pos] = cv[q];
-      }
+      for (int j = 0; j < NP; ++j) out.p[(long long)j * cap + pos] = V[u].p[j];
+      out.lam[pos] = V[u].lam;
+      out.f[pos] = V[u].f;
+      out.f_first[pos] = V[u].f_first;
+      out.st[pos] = (signed char)V[u].st;
+      out.cnt[pos] = cnt_total[u];
+      out.idx[pos] = (int)v[u];
+    } else if (valid[u]) {
+      store_result(a, v[u], V[u], cnt_total[u], cnt[u] > 0);
     }
-#pragma unroll
-    for (int j = 0; j < NP; ++j) out.p[(long long)j * cap + pos] = V.p[j];
-    out.lam[pos] = V.lam;
-    out.f[pos] = V.f;
-    out.f_first[pos] = V.f_first;
-    out.st[pos] = (signed char)V.st;
-    out.cnt[pos] = cnt_total;
-    out.idx[pos] = (int)v;
-  } else if (valid) {
-    store_result(a, v, V, cnt_total, cnt > 0);
+    base += __popcll(mask[u]);
   }
 }
 
@@ -544,20 +776,22 @@ PCA_MODEL_AT(3)
 #endif
 
 template <int MI>
-__device__ __forceinline__ void dispatch(int m, const pca_chunk_args& A, long long e, bool valid) {
+__device__ __forceinline__ void dispatch(int m, const pca_chunk_args& A, long long e0,
+                                         long long count) {
   if constexpr (MI < PCA_NMODELS) {
     if (m == MI) {
-      chunk_wave<MI, typename ModelAt<MI>::type, ModelAt<MI>::M, ModelAt<MI>::UNROLL>(A, e, valid);
+      chunk_wave<MI, typename ModelAt<MI>::type, ModelAt<MI>::M, ModelAt<MI>::UNROLL>(A, e0, count);
     } else {
-      dispatch<MI + 1>(m, A, e, valid);
+      dispatch<MI + 1>(m, A, e0, count);
     }
   }
 }
 
 }  // namespace pca
 
-// Grid-stride over 64-slot wave tasks; model m's queue occupies
-// ceil(count_m / 64) consecutive tasks, so every wavefront works on one model.
+// Grid-stride over wave tasks of 64 * PCA_VPL queue slots; model m's queue
+// occupies ceil(count_m / (64 * PCA_VPL)) consecutive tasks, so every
+// wavefront works on one model.
 extern "C" __global__ void __launch_bounds__(PCA_BLOCK) PCA_KATTR
 pca_lm_chunk(pca_chunk_args A) {
   // queue lengths: chunk 0 reads every model's voxels, later chunks the counts
@@ -569,7 +803,8 @@ pca_lm_chunk(pca_chunk_args A) {
   long long tasks[PCA_NMODELS + 1];
   tasks[0] = 0;
 #pragma unroll
-  for (int m = 0; m < PCA_NMODELS; ++m) tasks[m + 1] = tasks[m] + ((long long)cnt[m] + 63) / 64;
+  for (int m = 0; m < PCA_NMODELS; ++m)
+    tasks[m + 1] = tasks[m] + ((long long)cnt[m] + 64 * PCA_VPL - 1) / (64 * PCA_VPL);
   const long long total = tasks[PCA_NMODELS];
   const long long waves_per_block = PCA_BLOCK / 64;
   const long long nwaves = (long long)gridDim.x * waves_per_block;
@@ -579,8 +814,8 @@ pca_lm_chunk(pca_chunk_args A) {
 #pragma unroll
     for (int q = 1; q < PCA_NMODELS; ++q)
       if (t >= tasks[q]) m = q;
-    const long long e = (t - tasks[m]) * 64 + pca::lane_id();
-    pca::dispatch<0>(m, A, e, e < cnt[m]);
+    const long long e0 = (t - tasks[m]) * (64 * PCA_VPL) + pca::lane_id();
+    pca::dispatch<0>(m, A, e0, cnt[m]);
   }
 }
 

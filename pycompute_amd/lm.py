@@ -65,6 +65,9 @@ class LMSpec:
     # factored accumulation (with reassoc): parameter-only factors applied
     # once per iteration instead of once per measurement
     factored: bool = False
+    # voxels per lane of the chunked kernel (1 or 2; 64 * vpl voxels per
+    # wavefront): a lane iterates vpl voxels in lockstep
+    vpl: int = 1
 
     @property
     def nparam(self):
@@ -111,6 +114,10 @@ class LMProgram:
             raise ValueError("the strict build excludes fast_exp, fast_div and reassoc")
         if spec.factored and not spec.reassoc:
             raise ValueError("the factored accumulation needs reassoc")
+        if spec.vpl not in (1, 2):
+            raise ValueError(f"vpl must be 1 or 2, got {spec.vpl}")
+        if spec.vpl > 1 and spec.sched_group > 0:
+            raise ValueError("register fences (sched_group) apply to vpl = 1 only")
         options = list(BASE_OPTIONS)
         if spec.strict:
             options.append("-ffp-contract=off")
@@ -172,6 +179,8 @@ def build_source(spec: LMSpec, dag, members=()) -> str:
         head.append("#define PCA_FACTORED 1")
     if spec.sched_group > 0:
         head.append(f"#define PCA_SCHED_GROUP {spec.sched_group}")
+    if spec.vpl > 1:
+        head.append(f"#define PCA_VPL {spec.vpl}")
     structs, typedefs = {}, []
     for i, (s, d) in enumerate(models):
         sname = "PcaModel_" + hipgen.expr_hash(s.expr, 12)

@@ -144,7 +144,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
                  compaction: bool = True, chunk: int = 8, max_chunks: int = 0, unroll: int = 0,
                  waves_per_eu: int = 0, fast_div: bool = False, wide: bool | None = None,
                  tile: int = 136, reassoc: bool = False, sched_group: int = 0,
-                 factored: bool = False, **kwargs):
+                 factored: bool = False, vpl: int = 1, **kwargs):
         if (args and isinstance(args[0], str)) or "expr" in kwargs:
             # notebook form: (expr, pars_str, consts_str, ndata, dtype, write_to_file)
             a = _bind(args, kwargs, ("expr", "pars_str", "consts_str", "ndata", "dtype",
@@ -196,7 +196,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         self._chunk_state = None
         self._knobs = dict(unroll=int(unroll), waves_per_eu=int(waves_per_eu),
                            fast_div=bool(fast_div), reassoc=bool(reassoc),
-                           sched_group=int(sched_group), factored=bool(factored))
+                           sched_group=int(sched_group), factored=bool(factored),
+                           vpl=int(vpl))
         self.keep_workspace = (self.nparam <= 8) if keep_workspace is None else bool(keep_workspace)
         # wide: the kernels for models with more parameters than one lane's
         # registers hold (n > 8): csrc/lm_glm.hip for generalised linear
