@@ -21,6 +21,8 @@ of the caller's own problem:
                    at twice the registers per lane
   * waves_per_eu:  occupancy hint (mixed-model batches, batch_candidates())
   * tile:          wide-model kernel register/LDS split (wide_candidates())
+  * mfma:          generalised linear models with n > 8: the Hessian on the
+                   matrix cores (pca_lm_glm_mfma) instead of VALU tiles
 
 The fastest candidate whose results agree with the reference-rounding build on
 the sample (final SSR within 1e-4 relative on at least 98% of voxels, and no
@@ -125,7 +127,9 @@ def wide_candidates(allow_fast: bool = True):
                                       if allow_fast else [])
     for fe, fd, ra in fast:
         for tile in (100, 136, 170):
-            out.append({"tile": tile, "fast_exp": fe, "fast_div": fd, "reassoc": ra})
+            out.append({"tile": tile, "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                        "mfma": False})
+        out.append({"tile": 136, "fast_exp": fe, "fast_div": fd, "reassoc": ra, "mfma": True})
     return out
 
 

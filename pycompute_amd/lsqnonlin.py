@@ -144,7 +144,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
                  compaction: bool = True, chunk: int = 8, max_chunks: int = 0, unroll: int = 0,
                  waves_per_eu: int = 0, fast_div: bool = False, wide: bool | None = None,
                  tile: int = 136, reassoc: bool = False, sched_group: int = 0,
-                 factored: bool = False, vpl: int = 1, **kwargs):
+                 factored: bool = False, vpl: int = 1, mfma: bool = False, **kwargs):
         if (args and isinstance(args[0], str)) or "expr" in kwargs:
             # notebook form: (expr, pars_str, consts_str, ndata, dtype, write_to_file)
             a = _bind(args, kwargs, ("expr", "pars_str", "consts_str", "ndata", "dtype",
@@ -203,7 +203,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         # registers hold (n > 8): csrc/lm_glm.hip for generalised linear
         # models with voxel-uniform consts, csrc/lm_wide.hip (Hessian kept in
         # LDS) for any other model. None = use them whenever they apply.
-        self.wide, self.tile = wide, int(tile)
+        # mfma: the GLM kernel with the Hessian on the matrix cores (float32)
+        self.wide, self.tile, self.mfma = wide, int(tile), bool(mfma)
         self._glm, self._glm_tab, self._wide_prog = None, None, None
         # more consts than the kernel-argument block holds: per-voxel layout
         self._uc_fits = self.nconst * self.ndata <= _lib.PCA_UC_MAX
@@ -212,7 +213,10 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             self.funcid = self._spec_uniform.funcid()
             if wide is not False:
                 from . import wide as _wide
-                self._glm = _wide.glm_program(self._spec_uniform, self.tile)
+                self._glm = _wide.glm_program(self._spec_uniform, self.tile, self.mfma)
+                if self.mfma and self._glm is None:
+                    raise ValueError("mfma needs a float32 generalised linear model with n > 8 "
+                                     "whose tables fit in the LDS")
                 if wide and self._glm is None and _wide.wide_program(
                         self._spec(not self._uc_fits), self.tile) is None:
                     raise ValueError("the wide kernels need n > 8 and a packed Hessian that "

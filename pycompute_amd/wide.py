@@ -46,9 +46,24 @@ def lds_bytes(n: int, tile_max: int, dtype: str) -> int:
     return max(park_entries(n, tile_max), 1) * 64 * (8 if dtype == "float64" else 4)
 
 
+def mfma_lds_bytes(n: int, M: int) -> int:
+    """LDS of one 256-thread workgroup of pca_lm_glm_mfma: the entry-product
+    table (M rows of 4 x (groups padded to 4, plus 4) floats), the samples of
+    every voxel (4 wavefronts x M x 64 floats) and g parked over the solve
+    (4 x 64 x (n + 1) floats)."""
+    nq = (n * (n + 1) // 2 + 3) // 4
+    rs = (nq + 3) // 4 * 4 + 4
+    return M * 4 * rs * 4 + 4 * M * 64 * 4 + 4 * 64 * (n + 1) * 4
+
+
 class GLMProgram:
-    def __init__(self, spec: lm.LMSpec, tile_max: int = DEFAULT_TILE):
+    """pca_lm_glm (Hessian tiles accumulated on the VALU, parked in LDS) or,
+    with mfma=True, pca_lm_glm_mfma (Hessian accumulated on the matrix
+    cores, one voxel per lane; float only)."""
+
+    def __init__(self, spec: lm.LMSpec, tile_max: int = DEFAULT_TILE, mfma: bool = False):
         self.spec = spec
+        self.mfma = bool(mfma)
         self.dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
         self.form = glm.glm_form(self.dag)
         if self.form is None:
@@ -60,8 +75,14 @@ class GLMProgram:
         if self.lds_bytes > LDS_LIMIT:
             raise ValueError(f"tile {self.tile_max} parks {self.lds_bytes} bytes of LDS per "
                              f"workgroup (limit {LDS_LIMIT}); use a larger tile")
+        if self.mfma:
+            if spec.dtype != "float32":
+                raise ValueError("the matrix-core GLM kernel is float32 only")
+            if mfma_lds_bytes(n, spec.M) > LDS_LIMIT:
+                raise ValueError(f"the matrix-core GLM kernel needs {mfma_lds_bytes(n, spec.M)} "
+                                 f"bytes of LDS for M = {spec.M} (limit {LDS_LIMIT})")
         self.stride = glm.table_stride(n)
-        self.funcid = spec.funcid() + f"_glm{self.tile_max}"
+        self.funcid = spec.funcid() + f"_glm{self.tile_max}" + ("_mfma" if self.mfma else "")
         head = [f"// generated: {self.funcid}", f"// model: {spec.expr}",
                 f"#define PCA_NP {n}", f"#define PCA_M {spec.M}", f"#define PCA_TS {self.stride}",
                 f"#define PCA_TILE_MAX {self.tile_max}", f"#define PCA_BLOCK {spec.block}"]
@@ -75,7 +96,8 @@ class GLMProgram:
             head.append("#define PCA_FAST_SOLVE 1")
         self.source = "\n".join(["\n".join(head), read_csrc("strict_math.h"),
                                  read_csrc("prelude.hip"), read_csrc("lm_solver.hip"),
-                                 read_csrc("lm_glm.hip")])
+                                 read_csrc("lm_glm.hip")]
+                                + ([read_csrc("lm_glm_mfma.hip")] if self.mfma else []))
         options = list(BASE_OPTIONS)
         if spec.fast_div:
             options = [o for o in options if o != "-fhip-fp32-correctly-rounded-divide-sqrt"]
@@ -84,20 +106,53 @@ class GLMProgram:
         fl = hipgen.flops_per_iteration(self.dag, spec.M)
         self.flops = glm_flops(n, spec.M, fl)
 
-    def table(self, uc, device) -> torch.Tensor:
-        """Coefficient table for voxel-uniform consts (k*M+i ordering)."""
+    def table(self, uc, device):
+        """Coefficient table for voxel-uniform consts (k*M+i ordering); with
+        mfma, the pair (table, entry-product table)."""
         c = np.asarray(uc, np.float64).reshape(self.spec.nconst, self.spec.M)
         tab = glm.coefficient_table(self.form, self.dag, c)
-        return torch.tensor(tab, dtype=lm.torch_dtype(self.spec.dtype), device=device)
+        t = torch.tensor(tab, dtype=lm.torch_dtype(self.spec.dtype), device=device)
+        if not self.mfma:
+            return t
+        return t, torch.tensor(entry_products(tab, self.spec.nparam), dtype=torch.float32,
+                               device=device)
 
-    def launch(self, args: _lib.LMArgs, tab: torch.Tensor, stream=None):
+    def launch(self, args: _lib.LMArgs, tab, stream=None):
+        s = lm.stream_handle() if stream is None else stream
+        if self.mfma:
+            t, t2 = (ctypes.c_void_p(x.data_ptr()) for x in tab)
+            params = (ctypes.c_void_p * 3)(ctypes.cast(ctypes.pointer(args), ctypes.c_void_p),
+                                           ctypes.cast(ctypes.pointer(t), ctypes.c_void_p),
+                                           ctypes.cast(ctypes.pointer(t2), ctypes.c_void_p))
+            grid = (int(args.N) + 255) // 256
+            _lib.check(_lib.lib().pca_launch(self.module.kernel("pca_lm_glm_mfma"), grid, 1, 1,
+                                             256, 1, 1, 0, s, params), "pca_lm_glm_mfma")
+            return
         t = ctypes.c_void_p(tab.data_ptr())
         params = (ctypes.c_void_p * 2)(ctypes.cast(ctypes.pointer(args), ctypes.c_void_p),
                                        ctypes.cast(ctypes.pointer(t), ctypes.c_void_p))
         grid = (int(args.N) + 63) // 64
-        s = lm.stream_handle() if stream is None else stream
         _lib.check(_lib.lib().pca_launch(self.module.kernel("pca_lm_glm"), grid, 1, 1, 64, 1, 1,
                                          0, s, params), "pca_lm_glm")
+
+
+def entry_products(tab: np.ndarray, n: int) -> np.ndarray:
+    """(M, 4, RS) float64 table of pca_lm_glm_mfma: t2[i, r, q] = A_ij A_ik
+    of packed entry e = 4q + r = j(j+1)/2 + k at measurement i (0 past the
+    last entry), rows padded to RS = (groups rounded up to 4) + 4."""
+    M = tab.shape[0]
+    nh = n * (n + 1) // 2
+    nq = (nh + 3) // 4
+    rs = (nq + 3) // 4 * 4 + 4
+    prod = np.zeros((M, 4 * nq))
+    e = 0
+    for j in range(n):
+        for k in range(j + 1):
+            prod[:, e] = tab[:, j] * tab[:, k]
+            e += 1
+    out = np.zeros((M, 4, rs))
+    out[:, :, :nq] = prod.reshape(M, nq, 4).transpose(0, 2, 1)
+    return out
 
 
 def glm_flops(n: int, M: int, generic: dict) -> dict:
@@ -196,14 +251,19 @@ def wide_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> WideProgram |
         return prog
 
 
-def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> GLMProgram | None:
+def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE,
+                mfma: bool = False) -> GLMProgram | None:
     """The wide-kernel program for ``spec``, or None when the model is not a
     GLM, is small enough for the register kernel, or parks more LDS than a
-    workgroup may hold at this tile."""
+    workgroup may hold at this tile (with mfma: is not float32, or its
+    tables exceed the LDS)."""
     if (spec.nparam < WIDE_MIN_PARAMS or spec.strict
             or lds_bytes(spec.nparam, tile_max, spec.dtype) > LDS_LIMIT):
         return None
-    key = (spec, tile_max)
+    if mfma and (spec.dtype != "float32"
+                 or mfma_lds_bytes(spec.nparam, spec.M) > LDS_LIMIT):
+        return None
+    key = (spec, tile_max, bool(mfma))
     with _lock:
         if key in _programs:
             return _programs[key]
@@ -213,7 +273,7 @@ def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> GLMProgram | N
             if key in _programs:
                 return _programs[key]
         dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
-        prog = GLMProgram(spec, tile_max) if glm.glm_form(dag) else None
+        prog = GLMProgram(spec, tile_max, mfma) if glm.glm_form(dag) else None
         with _lock:
             _programs[key] = prog
         return prog

@@ -82,3 +82,31 @@ def test_wide_kernel_lds_budget():
     # 36 parameters: 666 entries, 170 KB in FP32, over the limit
     assert wide.wide_lds_bytes(36, "float32") > wide.LDS_LIMIT
     assert wide.lds_bytes(60, 40, "float32") > wide.LDS_LIMIT
+
+
+def test_mfma_entry_product_table_layout():
+    """pca_lm_glm_mfma's table: t2[i, r, q] = A_ij A_ik of packed entry
+    e = 4q + r = j(j+1)/2 + k, zero past the last entry, rows of
+    RS = (groups rounded up to 4) + 4 floats; checked against a direct loop
+    over (j, k) on the kurtosis design."""
+    from pycompute_amd import wide
+    consts = synth.dki_protocol()
+    A = dki_design(consts)
+    M, n = A.shape
+    tab = np.zeros((M, glm.table_stride(n)))
+    tab[:, :n] = A
+    t2 = wide.entry_products(tab, n)
+    nh = n * (n + 1) // 2
+    nq = (nh + 3) // 4
+    assert t2.shape == (M, 4, (nq + 3) // 4 * 4 + 4) == (61, 4, 68)
+    for j in range(n):
+        for k in range(j + 1):
+            e = j * (j + 1) // 2 + k
+            np.testing.assert_array_equal(t2[:, e % 4, e // 4], A[:, j] * A[:, k])
+    mask = np.ones(t2.shape, bool)
+    for e in range(nh):
+        mask[:, e % 4, e // 4] = False
+    assert np.all(t2[mask] == 0)
+    # LDS of a 256-thread workgroup: the table, the samples and g over the solve
+    assert wide.mfma_lds_bytes(22, 61) == 61 * 4 * 68 * 4 + 4 * 61 * 64 * 4 + 4 * 64 * 23 * 4
+    assert wide.mfma_lds_bytes(22, 61) <= wide.LDS_LIMIT < wide.mfma_lds_bytes(22, 80)
