@@ -526,9 +526,12 @@ def run_dki(args, stream):
         s.pars_t.copy_(p0s)
         s.reset_state()
 
-    kn = autotune.tune(make_sub, restore_sub, prob.model.expr, prob.M, "float32",
-                       allow_fast=not rr, log=log, iters=30, reps=2,
-                       cands=autotune.wide_candidates(not rr))
+    if args.dki_tuning:
+        kn = json.loads(args.dki_tuning)
+    else:
+        kn = autotune.tune(make_sub, restore_sub, prob.model.expr, prob.M, "float32",
+                           allow_fast=not rr, log=log, iters=30, reps=2,
+                           cands=autotune.wide_candidates(not rr))
     log(f"DKI wide kernel tuned: {kn}")
     solver = fit.make_solver(prob, **kn)
     pars = torch.empty((22, N), dtype=torch.float32, device=prob.data_t.device)
@@ -562,6 +565,16 @@ def run_dki(args, stream):
     tflops = ((its - conv) * fl["F_it"] + conv * fl["F_conv"]) / (k_ms * 1e-3) / 1e12
     log(f"DKI plain: {ms_plain:.2f} ms/step, LM kernel {k_ms:.2f} ms, {tflops:.2f} TFLOP/s, "
         f"{int((~ok).sum())} voxels excluded")
+    plain = {"ms_per_step": ms_plain, "value": N / (ms_plain * 1e-3),
+             "total_final_ssr": plain_ssr, "iterations_per_voxel": its / N,
+             "lm_kernel_ms": k_ms, "lm_tflops": tflops,
+             "lm_frac_of_fp32_peak": tflops * 1e12 / PEAK_FP32,
+             "excluded_voxels": int((~ok).sum())}
+    kernel = "pca_lm_glm_mfma (wide GLM kernel, Hessian on the matrix cores)" \
+        if kn.get("mfma") else "pca_lm_glm (wide GLM kernel)"
+    if args.dki_plain_only:
+        return {"workload": "DKI 22 parameters, 2,097,152 voxels, 61 measurements, SNR 30",
+                "voxels": N, "kernel": kernel, "tuning": kn, "plain_lm": plain}
 
     # the initialiser: trained on forward-model signals of a separate seed
     net = mlp.dki_mlp(w["consts"])
@@ -650,13 +663,8 @@ def run_dki(args, stream):
                                     g.cpu().numpy(), iters, r.pars_t.cpu().numpy())
     res["mlp_lm"]["fp64_oracle_error"] = fp64.get("mlp_lm")
     return {"workload": "DKI 22 parameters, 2,097,152 voxels, 61 measurements, SNR 30",
-            "voxels": N, "kernel": "pca_lm_glm (wide-model kernel)", "tuning": kn,
-            "plain_lm": {"ms_per_step": ms_plain, "value": N / (ms_plain * 1e-3),
-                         "total_final_ssr": plain_ssr, "iterations_per_voxel": its / N,
-                         "lm_kernel_ms": k_ms, "lm_tflops": tflops,
-                         "lm_frac_of_fp32_peak": tflops * 1e12 / PEAK_FP32,
-                         "excluded_voxels": int((~ok).sum()),
-                         "fp64_oracle_error": fp64.get("plain_lm")},
+            "voxels": N, "kernel": kernel, "tuning": kn,
+            "plain_lm": dict(plain, fp64_oracle_error=fp64.get("plain_lm")),
             **res, "speedup_vs_plain_lm": ms_plain / res["mlp_lm"]["ms_per_step"]}
 
 
@@ -683,6 +691,10 @@ def main():
     ap.add_argument("--no-dki", action="store_true", help="skip the kurtosis-model volume")
     ap.add_argument("--dki-train-samples", type=int, default=10_000_000)
     ap.add_argument("--dki-train-epochs", type=int, default=4)
+    ap.add_argument("--dki-tuning", type=str, default=None,
+                    help="wide-kernel knobs for the kurtosis leg as JSON (skips its autotuner)")
+    ap.add_argument("--dki-plain-only", action="store_true",
+                    help="kurtosis leg: the plain fit only (no network, no capped fits)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): voxel fits need no "

@@ -100,7 +100,7 @@ def candidates(nparam: int, allow_fast: bool):
 def batch_candidates(allow_fast: bool = True):
     """Knobs for a mixed-model batch: one kernel serves every model, so its
     register allocation is the largest model's; the occupancy hint (waves per
-    SIMD) trades that against spills."""
+    SIMD) trades that against spills, the register fence bounds it."""
     out = []
     fast = [(False, False, False, False)] + (
         [(True, True, False, False), (True, True, True, False), (True, True, True, True)]
@@ -115,6 +115,13 @@ def batch_candidates(allow_fast: bool = True):
         out.append({"block": 128, "chunk": 16, "max_chunks": 4, "unroll": 0,
                     "fast_exp": fe, "fast_div": fd, "reassoc": ra,
                     "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2})
+        if fac:
+            # a register fence every 4 measurements: 3 waves per SIMD
+            # instead of 2 for the three-model batch
+            for chunk, mc in ((16, 0), (16, 4)):
+                out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                            "waves_per_eu": 0, "factored": fac, "sched_group": 4, "vpl": 1})
     return out
 
 
