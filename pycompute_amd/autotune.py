@@ -88,12 +88,14 @@ def candidates(nparam: int, allow_fast: bool):
                             "fast_exp": fe, "fast_div": fd, "reassoc": ra,
                             "waves_per_eu": wpe, "factored": fac, "sched_group": sg,
                             "vpl": 1})
-        if sg == 0 and wpe == 0 and nparam <= 8:
+        if sg == 0 and nparam <= 8:
             # two voxels per lane (one wavefront per SIMD for most models)
             for chunk, mc in ((16, 0), (16, 4)):
-                out.append({"block": 128, "chunk": chunk, "max_chunks": mc, "unroll": 0,
-                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                            "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2})
+                c = {"block": 128, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                     "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                     "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2}
+                if c not in out:
+                    out.append(c)
     return out
 
 
