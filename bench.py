@@ -182,7 +182,7 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
     from pycompute_amd import fit, mlp, synth
 
     N = prob.N
-    net = mlp.ivim_mlp(synth.B_C2, hidden=128, seed=0)
+    net = mlp.ivim_mlp(synth.B_C2, hidden=args.mlp_hidden, seed=0)
     t0 = time.perf_counter()
     data, truth = net.synthesize(args.train_samples, seed=20240613, snr=(20.0, 100.0),
                                  lo=IVIM_BOX[0], hi=IVIM_BOX[1])
@@ -280,7 +280,8 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
     e1.record(stream)
     torch.cuda.synchronize()
     infer_ms = e0.elapsed_time(e1) / 20
-    mlp_flops = 2.0 * N * (16 * 128 + 128 * 128 + 128 * 4)
+    H = net.H
+    mlp_flops = 2.0 * N * (16 * H + H * H + H * 4)
     # LM kernels of the MLP-initialised fit alone
     lm_ms = []
     for _ in range(3):
@@ -681,6 +682,8 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="skip the MLP-initialised fit")
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--mlp-hidden", type=int, default=64, choices=(32, 64, 128, 256),
+                    help="hidden width of the IVIM initialiser")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-model batch")
     ap.add_argument("--tuning", type=str, default=None,
                     help="JSON knob set for the IVIM fit instead of the autotuner's choice")

@@ -646,9 +646,11 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
   if (args->N > 0 && (!args->data || !args->params || !args->lb || !args->ub))
     return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: NULL argument");
   const int H = args->hidden, IN = args->in, OUT = args->out;
-  if (IN < 1 || IN > 64 || OUT < 1 || OUT > 32 || (H != 128 && H != 256) || args->N < 0 ||
-      args->norm_index < 0 || args->norm_index >= IN)
-    return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: sizes must be in <= 64, out <= 32, hidden 128|256");
+  if (IN < 1 || IN > 64 || OUT < 1 || OUT > 32 ||
+      (H != 32 && H != 64 && H != 128 && H != 256) || args->N < 0 || args->norm_index < 0 ||
+      args->norm_index >= IN || (H < 128 && (IN > 16 || OUT > 16)))
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_infer: sizes must be in <= 64, out <= 32, hidden "
+                                 "32|64|128|256 (32, 64: in <= 16, out <= 16)");
   if (args->fallback_P) {
     const int K = args->fallback_K, kind = args->fallback_kind;
     if ((kind == 0 && K != OUT) || (kind == 1 && (K != 2 || OUT != 2)) ||
@@ -685,7 +687,11 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
     else if (OUT == 4) PCA_INFER(HH, II, OO, VT, L, 4);                   \
     else PCA_INFER(HH, II, OO, VT, L, 0);                                 \
   } while (0)
-  if (!big) {
+  if (H == 32) {
+    PCA_INFER_OUT(32, 16, 16, 4, 1);   // the narrow IVIM-sized networks
+  } else if (H == 64) {
+    PCA_INFER_OUT(64, 16, 16, 4, 1);
+  } else if (!big) {
     if (INP == 16 && OUTP == 16) PCA_INFER_OUT(128, 16, 16, 4, 1);
     else if (INP == 16) PCA_INFER(128, 16, 32, 4, 0, 0);
     else if (INP == 32 && OUTP == 16) PCA_INFER_OUT(128, 32, 16, 4, 1);
