@@ -119,7 +119,8 @@ typedef struct pca_ws {
 } pca_ws;
 
 /* One model's part of a chunked launch: its caller arrays, its voxel-uniform
- * consts (device pointer, NULL for per-voxel consts) and two queues. */
+ * consts table (device pointer, laid out as pca_lm_launch's uconsts; NULL for
+ * per-voxel consts) and two queues. */
 typedef struct pca_model_job {
   pca_lm_args a;
   const void* uc;
@@ -158,8 +159,11 @@ int pca_module_kernel(pca_module module, const char* name, pca_kernel* out);
 
 /* Launch the fused LM fit (kernel "pca_lm_fit" of a generated module) over
  * ceil(N/block) blocks; `block` must equal the block size the module was
- * generated for. `uconsts` holds the voxel-uniform consts (uconsts_bytes <=
- * PCA_UC_MAX * element size; NULL for per-voxel modules). Replaces the
+ * generated for. `uconsts` holds the voxel-uniform consts table (uconsts_bytes <=
+ * PCA_UC_MAX * element size; NULL for per-voxel modules): nconst*M values,
+ * const k of measurement i at k*M+i, then, for modules of the factored fast
+ * build, the rows of derived consts that the generated source lists in its
+ * head comment (const-only subexpressions evaluated once per protocol). Replaces the
  * per-iteration launch loop of SecondOrderLevenbergMarquardt.run
  * (pycompute/cuda/lsqnonlin.py:1058-1073). */
 int pca_lm_launch(pca_kernel kernel, const pca_lm_args* args, const void* uconsts,
@@ -199,7 +203,8 @@ int64_t pca_static_scratch_bytes(void);
  * for callers that cannot allocate (code under graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). The caller passes its inputs and
  * outputs: one pca_lm_args per model (as in pca_model_job.a), the device
- * pointer of each model's voxel-uniform consts (or NULL), each model's
+ * pointer of each model's voxel-uniform consts table (as pca_lm_launch's
+ * uconsts, derived rows included; or NULL), each model's
  * parameter count, and is_double for the element type. Two queues of N
  * entries per model must fit: 2 * N * ((n + 3) * es + 9) bytes plus 256-byte
  * alignment per array (4-parameter float32 IVIM: about 1.36M voxels per

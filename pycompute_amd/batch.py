@@ -102,7 +102,8 @@ class BatchedLM:
         per model, in the reference's layouts; pars_t is updated in place."""
         if len(problems) != len(self.members):
             raise ValueError(f"expected {len(self.members)} problems, got {len(problems)}")
-        for mem, (pars_t, consts_t, data_t, lb_t, ub_t) in zip(self.members, problems):
+        for mi, (mem, (pars_t, consts_t, data_t, lb_t, ub_t)) in enumerate(
+                zip(self.members, problems)):
             s = mem.spec
             for t, nm in ((pars_t, "pars_t"), (data_t, "data_t"), (lb_t, "lower_bound_t"),
                           (ub_t, "upper_bound_t"), (consts_t, "consts_t")):
@@ -121,7 +122,8 @@ class BatchedLM:
             dev = mem.data_t.device
             kw = dict(dtype=self.dtype, device=dev)
             mem.uc = uc
-            mem.uc_t = torch.tensor(uc, **kw)
+            # the chunked kernel reads the model's derived consts rows too
+            mem.uc_t = torch.tensor(self.program.uconsts(uc, mi), **kw)
             mem.lam_t = torch.full((1, N), 5.0, **kw)
             mem.step_type_t = torch.ones((1, N), dtype=torch.int8, device=dev)
             mem.f_t = torch.zeros((1, N), **kw)

@@ -50,7 +50,9 @@
 #endif
 // Factored accumulation (the model struct's prep/accf/finish, rprep/residf):
 // parameter-only factors of the Jacobian and second derivatives are applied
-// once per iteration instead of once per measurement.
+// once per iteration instead of once per measurement, and const-only
+// subexpressions are read as consts Mdl::NC .. Mdl::NCT-1 of the measurement
+// (rows of the voxel-uniform consts table filled by the host).
 #ifndef PCA_FACTORED
 #define PCA_FACTORED 0
 #endif
@@ -110,9 +112,9 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
     for (int j = 0; j < NP; ++j) dmp[j] = 0;
 #pragma unroll UNROLL
     for (int i = 0; i < M; ++i) {
-      real c[NC];
+      real c[Mdl::NCT];
 #pragma unroll
-      for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+      for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(k, i);
       Mdl::accf(P, c, V.y[i], V.f, s, q, dmp);
       if constexpr (PCA_SCHED_GROUP > 0) {
         if (i % PCA_SCHED_GROUP == PCA_SCHED_GROUP - 1 && i + 1 < M) {
@@ -192,9 +194,9 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
     Mdl::rprep(ptp, R);
 #pragma unroll UNROLL
     for (int i = 0; i < M; ++i) {
-      real c[NC];
+      real c[Mdl::NCT];
 #pragma unroll
-      for (int k = 0; k < NC; ++k) c[k] = cf(k, i);
+      for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(k, i);
       const real r = Mdl::residf(R, c, V.y[i]);
       ftp += r * r;
       if constexpr (PCA_SCHED_GROUP > 0) {
@@ -295,9 +297,9 @@ __device__ __forceinline__ void iterate_n(Voxel<Mdl, M> (&V)[VPL], const Setting
     for (int i = 0; i < M; ++i) {
 #pragma unroll
       for (int u = 0; u < VPL; ++u) {
-        real c[NC];
+        real c[Mdl::NCT];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) c[k] = cf(u, k, i);
+        for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(u, k, i);
         Mdl::accf(P[u], c, V[u].y[i], f[u], s[u], q[u], dmp[u]);
       }
     }
@@ -371,9 +373,9 @@ __device__ __forceinline__ void iterate_n(Voxel<Mdl, M> (&V)[VPL], const Setting
     for (int i = 0; i < M; ++i) {
 #pragma unroll
       for (int u = 0; u < VPL; ++u) {
-        real c[NC];
+        real c[Mdl::NCT];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) c[k] = cf(u, k, i);
+        for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(u, k, i);
         const real r = Mdl::residf(R[u], c, V[u].y[i]);
         ftp[u] += r * r;
       }
@@ -485,7 +487,10 @@ __device__ __forceinline__ void load_user(const pca_lm_args& a, long long v, Vox
 }  // namespace pca
 
 #ifndef PCA_CONSTS_PER_VOXEL
-static_assert(Model::NC * PCA_M <= PCA_UC_MAX, "consts exceed the kernel-argument block");
+// NCT - NC rows of derived consts follow the model's own (factored builds)
+static_assert(Model::NCT * PCA_M <= PCA_UC_MAX, "consts exceed the kernel-argument block");
+#else
+static_assert(Model::NCT == Model::NC, "derived consts need voxel-uniform consts");
 #endif
 
 // Per-voxel consts: kept in registers when there are at most this many.

@@ -64,6 +64,13 @@ __device__ __forceinline__ double pca_pow(double x, double y) { return pow(x, y)
 #else
 #define PCA_EXPK(k, x) PCA_EXP((k) * (x))
 #endif
+// exp(k*x) with k read from the derived consts, already scaled by log2(e)
+// on the host when the fast-exp float build evaluates exp through exp2.
+#if defined(PCA_FAST_EXP) && !defined(PCA_REAL_DOUBLE)
+#define PCA_EXPKD(ks, x) __builtin_amdgcn_exp2f((ks) * (x))
+#else
+#define PCA_EXPKD(ks, x) PCA_EXP((ks) * (x))
+#endif
 
 __device__ __forceinline__ float pca_fmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double pca_fmax(double a, double b) { return fmax(a, b); }

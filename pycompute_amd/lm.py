@@ -10,6 +10,7 @@ import ctypes
 import threading
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -109,7 +110,10 @@ class LMProgram:
             ids = "|".join(m.funcid() for m in (spec,) + self.members)
             self.funcid = f"batched_levenberg_marquardt_{len(ids.split('|'))}_" \
                           f"{hipgen.expr_hash(ids, 12)}"
-        self.source = build_source(spec, self.dag, list(zip(self.members, self.member_dags)))
+        # per model: the derived consts rows its factored code reads
+        self.derived: list = []
+        self.source = build_source(spec, self.dag, list(zip(self.members, self.member_dags)),
+                                   self.derived)
         if spec.strict and (spec.fast_exp or spec.fast_div or spec.reassoc):
             raise ValueError("the strict build excludes fast_exp, fast_div and reassoc")
         if spec.factored and not spec.reassoc:
@@ -134,6 +138,22 @@ class LMProgram:
     def nhes(self):
         return self.dag.nhes
 
+    def uconsts(self, consts_flat, model: int = 0) -> list:
+        """The voxel-uniform consts table of model ``model`` as its kernels
+        read it: the NC*M given values (k*M+i ordering) followed by the rows
+        of its derived consts, evaluated in float64."""
+        vals = [float(x) for x in consts_flat]
+        ent = self.derived[model] if self.derived else []
+        if not ent:
+            return vals
+        s = (self.spec,) + self.members
+        dag = (self.dag, *self.member_dags)[model]
+        M = s[model].M
+        rows = np.asarray(vals, dtype=np.float64).reshape(dag.nconst, M)
+        exp2 = self.spec.fast_exp and self.spec.dtype == "float32"
+        return vals + [float(x) for x in
+                       hipgen.derived_values(ent, dag, rows, exp2).ravel()]
+
     def kernel(self, name: str):
         return self.module.kernel(name)
 
@@ -144,10 +164,18 @@ def _unroll(spec: LMSpec) -> int:
     return spec.unroll if spec.unroll > 0 else (spec.M if spec.nparam <= 8 else 1)
 
 
-def build_source(spec: LMSpec, dag, members=()) -> str:
+def derive_consts(spec: LMSpec) -> bool:
+    """Whether the module reads const-only subexpressions of its factored
+    accumulation from derived consts rows (voxel-uniform consts only)."""
+    return spec.factored and not spec.consts_per_voxel
+
+
+def build_source(spec: LMSpec, dag, members=(), derived: list | None = None) -> str:
     """HIP source of one module. ``members``: further (LMSpec, dag) pairs that
     share the module's pca_lm_chunk kernel as Model1, Model2, ... (the
-    launch/code-generation knobs of ``spec`` apply to all of them)."""
+    launch/code-generation knobs of ``spec`` apply to all of them). A list
+    as ``derived`` receives, per model, the entries of its derived consts
+    (hipgen.DerivedConsts; empty when it has none)."""
     models = [(spec, dag)] + list(members)
     head = [f"// generated: {spec.funcid()}"]
     for i, (s, _) in enumerate(models):
@@ -181,12 +209,31 @@ def build_source(spec: LMSpec, dag, members=()) -> str:
         head.append(f"#define PCA_SCHED_GROUP {spec.sched_group}")
     if spec.vpl > 1:
         head.append(f"#define PCA_VPL {spec.vpl}")
-    structs, typedefs = {}, []
+    structs, typedefs, entries = {}, [], {}
+    maxM: dict = {}
+    for s, _ in models:
+        sname = "PcaModel_" + hipgen.expr_hash(s.expr, 12)
+        maxM[sname] = max(maxM.get(sname, 0), s.M)
     for i, (s, d) in enumerate(models):
         sname = "PcaModel_" + hipgen.expr_hash(s.expr, 12)
         if sname not in structs:
-            structs[sname] = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc,
-                                                factored=spec.factored)
+            ent = [] if derive_consts(spec) else None
+            txt = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc,
+                                    factored=spec.factored, derived=ent)
+            if ent and (d.nconst + len(ent)) * maxM[sname] > _lib.PCA_UC_MAX:
+                # the derived rows would not fit the consts block: computed
+                # per measurement instead
+                ent = None
+                txt = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc,
+                                        factored=spec.factored)
+            structs[sname], entries[sname] = txt, ent or []
+        for r, (kind, node) in enumerate(entries[sname]):
+            what = "exp factor, times log2(e) in fast-exp float builds" \
+                if kind == "expk" else "value"
+            head.append(f"// model {i} derived const c[{d.nconst + r}] ({what}): "
+                        f"{hipgen.E.to_str(node)}")
+        if derived is not None:
+            derived.append(entries[sname])
         if members:
             typedefs.append(f"typedef {sname} Model{i};")
         if i == 0:

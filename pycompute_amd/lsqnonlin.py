@@ -308,8 +308,11 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         if (self._cpv is not None) != self.program_is_per_voxel():
             self.build(consts_per_voxel=self._cpv is not None)
         if self._uc is not None:
-            self._ucbuf, self._ucbytes = lm.uconsts_buffer(self._uc, self.dtype_name)
+            # the given consts and the derived rows the generated code reads
+            self._ucx = self.program.uconsts(self._uc)
+            self._ucbuf, self._ucbytes = lm.uconsts_buffer(self._ucx, self.dtype_name)
         else:
+            self._ucx = None
             self._ucbuf, self._ucbytes = None, 0
         self._glm_tab = None
         if self._glm is not None and self._uc is not None:
@@ -403,8 +406,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             ncm = self.nconst * self.ndata if self._cpv is not None else 0
             self._ws = chunked.Workset(self.Nprobs, self.nparam, self.dtype, dev, M=self.ndata,
                                        ncm_per_voxel=ncm)
-            if self._uc is not None:
-                self._uc_t = torch.tensor(self._uc, dtype=self.dtype, device=dev)
+            if self._ucx is not None:
+                self._uc_t = torch.tensor(self._ucx, dtype=self.dtype, device=dev)
         a = self.lm_args(0, tol, mu, eta, acc, dec)
         r = chunked.ChunkedRun(self.program, [(self.program, a, self._uc_t, self._ws)], dev,
                                self.block)

@@ -247,18 +247,68 @@ class _Factorer:
         return self.g.mul(self.g.num(coef), self.prod(P)), self.prod(K + U)
 
 
+class DerivedConsts:
+    """Const-only subexpressions of the per-measurement code, evaluated once
+    per protocol on the host and read as extra consts c[NC + i] (rows of the
+    voxel-uniform consts table after the model's own NC rows).
+
+    entries[i] = (kind, node): kind "val" is the node's value; kind "expk"
+    is the factor k of an exp(k*x), scaled by log2(e) when the fast-exp
+    float build evaluates exp through exp2 (PCA_EXPKD)."""
+
+    def __init__(self, dag: ModelDAG, fac: "_Factorer"):
+        self.nc = dag.nconst
+        self.fac = fac
+        self.index: dict = {}
+        self.entries: list = []
+
+    def hoists(self, n: E.Node) -> bool:
+        """An interior node over consts only that costs an instruction per
+        measurement (a negated const is a free source modifier)."""
+        if n.op in ("num", "sym") or (n.op == "neg" and n.args[0].op == "sym"):
+            return False
+        return self.fac.fs(n) <= self.fac.consts
+
+    def ref(self, kind: str, n: E.Node) -> str:
+        i = self.index.get((kind, n.id))
+        if i is None:
+            i = self.index[(kind, n.id)] = len(self.entries)
+            self.entries.append((kind, n))
+        return f"c[{self.nc + i}]"
+
+
+def derived_values(entries, dag: ModelDAG, consts_rows, exp2_scale: bool):
+    """Values of the DerivedConsts entries for one protocol, in float64:
+    consts_rows is (NC, M), one row per const; returns (len(entries), M)."""
+    C = np.asarray(consts_rows, dtype=np.float64).reshape(dag.nconst, -1)
+    M = C.shape[1]
+    env = dict(zip(dag.consts, C))
+    memo: dict = {}
+    out = np.empty((len(entries), M))
+    for i, (kind, n) in enumerate(entries):
+        v = np.broadcast_to(np.asarray(E.evaluate(n, env, memo), dtype=np.float64), (M,))
+        out[i] = v * (1.0 / np.log(2.0) if kind == "expk" and exp2_scale else 1.0)
+    return out
+
+
 class _MeasPrinter(_Printer):
     """Per-measurement code of the factored accumulation: parameter-only
     subexpressions are read from the per-iteration table ``{pre}.a[i]``
     (collected in ``pre_nodes``), and exp(k*x) with a const-only factor k is
-    printed as PCA_EXPK(k, x) so that the scaled k is loop-invariant."""
+    printed as PCA_EXPK(k, x) so that the scaled k is loop-invariant. With a
+    DerivedConsts registry, const-only subexpressions are read from the
+    consts table instead and exp(k*x) becomes PCA_EXPKD(scaled k, x)."""
 
-    def __init__(self, dag, dtype, fac: _Factorer, pre: str):
+    def __init__(self, dag, dtype, fac: _Factorer, pre: str, derived: DerivedConsts | None = None):
         super().__init__(dag, dtype)
         self.fac, self.pre_name = fac, pre
         self.pre: dict = {}
         self.pre_nodes: list = []
         self._expk: dict = {}
+        self.derived = derived
+
+    def _hoisted(self, n: E.Node) -> bool:
+        return self.derived is not None and self.derived.hoists(n)
 
     def ref(self, n: E.Node) -> str:
         if self.fac.param_only(n):
@@ -267,10 +317,12 @@ class _MeasPrinter(_Printer):
                 i = self.pre[n.id] = len(self.pre_nodes)
                 self.pre_nodes.append(n)
             return f"{self.pre_name}.a[{i}]"
+        if self._hoisted(n):
+            return self.derived.ref("val", n)
         return super().ref(n)
 
     def _children(self, n: E.Node):
-        if n.op in ("num", "sym") or self.fac.param_only(n):
+        if n.op in ("num", "sym") or self.fac.param_only(n) or self._hoisted(n):
             return ()
         if n.op == "div" and self.fac.param_only(n.args[1]):
             return (n.args[0],)
@@ -281,7 +333,7 @@ class _MeasPrinter(_Printer):
                 k = g.mul(g.num(coef), self.fac.prod(K))
                 x = g.mul(self.fac.prod(P), self.fac.prod(U))
                 self._expk[n.id] = (k, x)
-                return (k, x)
+                return (x,) if self.derived is not None else (k, x)
         return n.args
 
     def emit(self, outputs):
@@ -291,11 +343,15 @@ class _MeasPrinter(_Printer):
             while stack:
                 node, done = stack.pop()
                 if done:
-                    if node.op in ("num", "sym") or self.fac.param_only(node):
+                    if (node.op in ("num", "sym") or self.fac.param_only(node)
+                            or self._hoisted(node)):
                         continue
                     if node.id in self._expk:
                         k, x = self._expk[node.id]
-                        txt = f"PCA_EXPK({self.ref(k)}, {self.ref(x)})"
+                        if self.derived is not None:
+                            txt = f"PCA_EXPKD({self.derived.ref('expk', k)}, {self.ref(x)})"
+                        else:
+                            txt = f"PCA_EXPK({self.ref(k)}, {self.ref(x)})"
                     elif node.op == "div" and self.fac.param_only(node.args[1]):
                         # a parameter-only divisor: multiply by its reciprocal,
                         # computed once per iteration in the table
@@ -314,8 +370,9 @@ class _MeasPrinter(_Printer):
                         stack.append((c, False))
 
 
-def _emit_factored(dag: ModelDAG, dtype: str) -> list:
-    """Members of the factored accumulation (fast builds).
+def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
+    """Members of the factored accumulation (fast builds), and the
+    DerivedConsts entries they read (empty unless ``derive``).
 
     Every Jacobian entry and second derivative X is split as X = A(p) * u,
     A the product of its parameter-only factors (and sign), u the rest. The
@@ -352,7 +409,8 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
         return f"pca_fma({pr.ref(mA)}, {pr.ref(mu)}, -y)"
 
     # accf: one measurement
-    pm = _MeasPrinter(dag, dtype, fac, "P")
+    dc = DerivedConsts(dag, fac) if derive else None
+    pm = _MeasPrinter(dag, dtype, fac, "P", dc)
     pm.emit([mu] + [u for _, u in jf] + [x[1] for x in hf if x is not None])
     acc = list(pm.lines)
     acc.append(f"const real r = {resid(pm)};")
@@ -399,7 +457,7 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
             fin.append(f"h[{l}] = pca_fma({pf.ref(A2)}, s[{sidx[u2.id][0]}], {gn});")
 
     # trial residual: rprep once per trial point, residf per measurement
-    pr = _MeasPrinter(dag, dtype, fac, "R")
+    pr = _MeasPrinter(dag, dtype, fac, "R", dc)
     pr.emit([mu])
     rf = list(pr.lines) + [f"return {resid(pr)};"]
     prr = _Printer(dag, dtype)
@@ -407,7 +465,9 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
     rp = list(prr.lines) + [f"R.a[{i}] = {prr.ref(x)};" for i, x in enumerate(pr.pre_nodes)]
 
     ind = lambda lines: ["    " + x for x in lines]
+    nd = len(dc.entries) if dc is not None else 0
     out = ["  static constexpr bool FACTORED = true;",
+           f"  static constexpr int NCT = {dag.nconst + nd};",
            f"  static constexpr int NS = {max(1, len(sidx))};",
            f"  struct Pre {{ real a[{max(1, len(pm.pre_nodes))}]; real L[{n}]; }};",
            f"  struct RPre {{ real a[{max(1, len(pr.pre_nodes))}]; }};",
@@ -425,11 +485,12 @@ def _emit_factored(dag: ModelDAG, dtype: str) -> list:
     out += ["  __device__ static __forceinline__ real residf(const RPre& R, "
             "const real* __restrict__ c, real y) {"]
     out += ind(rf) + ["  }"]
-    return out
+    return out, (dc.entries if dc is not None else [])
 
 
 def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
-               reassoc: bool = False, factored: bool = False) -> str:
+               reassoc: bool = False, factored: bool = False,
+               derived: list | None = None) -> str:
     """HIP source of the model struct (see module docstring).
 
     reassoc=False keeps the reference's association of the per-measurement
@@ -439,7 +500,11 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
     kernel adds the diagonal of h after the measurement loop,
     HLD_DAMP_ONLY): fewer instructions, results equal up to rounding.
     factored=True (with reassoc) adds the factored accumulation
-    (_emit_factored), FACTORED = true."""
+    (_emit_factored), FACTORED = true. With a list as ``derived`` (factored
+    builds with voxel-uniform consts), the accumulation reads its const-only
+    subexpressions from consts rows NC .. NCT-1, and the list receives their
+    DerivedConsts entries (derived_values() computes the rows); NCT = NC
+    otherwise."""
     if factored and not reassoc:
         raise ValueError("the factored accumulation is a reassociated (fast) form")
     n, nh = dag.nparam, dag.nhes
@@ -510,8 +575,14 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
                "const real* __restrict__ c, real& m, real* J, real* d2) {")
     out += ["    " + s for s in body]
     out.append("  }")
-    out += _emit_factored(dag, dtype) if factored else \
-        ["  static constexpr bool FACTORED = false;"]
+    if factored:
+        lines, entries = _emit_factored(dag, dtype, derive=derived is not None)
+        out += lines
+        if derived is not None:
+            derived.extend(entries)
+    else:
+        out += ["  static constexpr bool FACTORED = false;",
+                f"  static constexpr int NCT = {dag.nconst};"]
     out.append("};")
     return "\n".join(out) + "\n"
 

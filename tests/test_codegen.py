@@ -263,6 +263,195 @@ def test_factored_sums_equal_fghhl(expr, pars, consts, lam):
     np.testing.assert_allclose(hld, hld_ref, rtol=1e-10, atol=1e-12)
 
 
+def _fghhl_ref(dag, p, cs, y, lam):
+    """f, g, h and hld of the reference's fghhl loop (lsqnonlin.py:237-256)
+    in float64, straight from the DAG."""
+    env = dict(zip(dag.pars, p))
+    env.update(zip(dag.consts, cs))
+    M = len(y)
+    ev = lambda node: np.broadcast_to(np.asarray(E.evaluate(node, env, {}), float), (M,))
+    r = ev(dag.model) - y
+    J = [ev(j) for j in dag.jac]
+    d2 = [ev(h) for h in dag.hes]
+    n = dag.nparam
+    g = np.array([np.sum(J[j] * r) for j in range(n)])
+    h, hld, l = [], np.zeros(n), 0
+    for j in range(n):
+        for k in range(j + 1):
+            h.append(np.sum(d2[l] * r + J[j] * J[k]))
+            if j == k:
+                hld[j] = np.sum(d2[l] * r + J[j] * J[k] + np.maximum(lam * J[j] ** 2, 1e-4))
+            l += 1
+    return float(np.sum(r * r)), g, np.array(h), hld
+
+
+_FACTORED_SHIM = """
+#include <cmath>
+#define __device__
+#define __forceinline__ inline
+typedef {real} real;
+static inline real pca_fma(real a, real b, real c) {{ return std::fma(a, b, c); }}
+static inline real pca_fmax(real a, real b) {{ return std::fmax(a, b); }}
+static inline real pca_floor_excess(real L, real uu) {{
+  return std::fmax(std::fma(-L, uu, (real)1e-4), (real)0);
+}}
+#define PCA_EXP(x) std::exp(x)
+#define PCA_LOG(x) std::log(x)
+#define PCA_POW(x, y) std::pow(x, y)
+#define PCA_EXPK(k, x) std::exp((k) * (x))
+#define PCA_EXPKD(ks, x) {expkd}
+{plain}
+{derived}
+// one iteration's sums as the kernel forms them: prep, accf per measurement
+// (consts k of measurement i at C[k*M+i]), finish, then hld += h_jj
+// (HLD_DAMP_ONLY) and the SSR through rprep/residf at the same point
+template <class Mdl>
+static void drive(const real* p, const real* C, const real* y, int M, real lam, real* out) {{
+  constexpr int NP = Mdl::NP, NH = Mdl::NH;
+  typename Mdl::Pre P;
+  Mdl::prep(p, lam, P);
+  real s[Mdl::NS] = {{}}, q[NH] = {{}}, dmp[NP] = {{}}, f = 0;
+  for (int i = 0; i < M; ++i) {{
+    real c[Mdl::NCT + 1];
+    for (int k = 0; k < Mdl::NCT; ++k) c[k] = C[k * M + i];
+    Mdl::accf(P, c, y[i], f, s, q, dmp);
+  }}
+  real g[NP], h[NH], hld[NP];
+  Mdl::finish(p, P.L, s, q, dmp, g, h, hld);
+  for (int j = 0; j < NP; ++j) hld[j] += h[j * (j + 1) / 2 + j];
+  typename Mdl::RPre R;
+  Mdl::rprep(p, R);
+  real fr = 0;
+  for (int i = 0; i < M; ++i) {{
+    real c[Mdl::NCT + 1];
+    for (int k = 0; k < Mdl::NCT; ++k) c[k] = C[k * M + i];
+    const real r = Mdl::residf(R, c, y[i]);
+    fr += r * r;
+  }}
+  int o = 0;
+  out[o++] = f;
+  out[o++] = fr;
+  for (int j = 0; j < NP; ++j) out[o++] = g[j];
+  for (int l = 0; l < NH; ++l) out[o++] = h[l];
+  for (int j = 0; j < NP; ++j) out[o++] = hld[j];
+}}
+extern "C" void run_plain(const real* p, const real* C, const real* y, int M, real lam,
+                          real* out) {{ drive<Plain>(p, C, y, M, lam, out); }}
+extern "C" void run_derived(const real* p, const real* C, const real* y, int M, real lam,
+                            real* out) {{ drive<Derived>(p, C, y, M, lam, out); }}
+"""
+
+
+def _build_factored_host(dag, dtype, exp2, tmp_path):
+    ent = []
+    plain = hipgen.emit_model(dag, "Plain", dtype, reassoc=True, factored=True)
+    derived = hipgen.emit_model(dag, "Derived", dtype, reassoc=True, factored=True,
+                                derived=ent)
+    real = "float" if dtype == "float32" else "double"
+    expkd = "std::exp2((ks) * (x))" if exp2 else "std::exp((ks) * (x))"
+    src = _FACTORED_SHIM.format(real=real, expkd=expkd, plain=plain, derived=derived)
+    cpp, so = tmp_path / "factored.cpp", tmp_path / "factored.so"
+    cpp.write_text(src)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-o", str(so), str(cpp)],
+                   check=True)
+    return ctypes.CDLL(str(so)), ent
+
+
+@pytest.mark.parametrize("expr,pars,consts", FACTOR_EXPRS)
+def test_factored_emitted_code_matches_fghhl(expr, pars, consts, tmp_path):
+    """The emitted factored members (prep/accf/finish, rprep/residf), built
+    for the host in float64 and driven as the kernel drives them, give the
+    reference's f, g, h, hld and trial SSR, both computing const-only
+    subexpressions per measurement and reading them as derived consts rows
+    (hipgen.derived_values) after the model's own consts."""
+    dag = hipgen.res_jac_hes(expr, pars, consts)
+    L, ent = _build_factored_host(dag, "float64", False, tmp_path)
+    rng = np.random.default_rng(11)
+    M, n, nc = 12, dag.nparam, dag.nconst
+    cs = np.array([rng.uniform(0.2, 2.0, M) for _ in dag.consts]).reshape(nc, M)
+    table = np.concatenate([cs, hipgen.derived_values(ent, dag, cs, False)]).ravel()
+    if nc == 0:
+        table = np.zeros(1)
+    dp = ctypes.POINTER(ctypes.c_double)
+    for lam in (1e-6, 0.37, 1e3):
+        p = np.array([rng.uniform(0.3, 1.5) for _ in dag.pars])
+        y = rng.uniform(0.0, 2.0, M)
+        f, g, h, hld = _fghhl_ref(dag, p, list(cs), y, lam)
+        want = np.concatenate([[f, f], g, h, hld])
+        for fn, C in ((L.run_plain, cs.ravel() if nc else np.zeros(1)), (L.run_derived, table)):
+            out = np.zeros(2 + 2 * n + dag.nhes)
+            fn.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_double, dp]
+            fn(p.ctypes.data_as(dp), np.ascontiguousarray(C).ctypes.data_as(dp),
+               y.ctypes.data_as(dp), M, lam, out.ctypes.data_as(dp))
+            np.testing.assert_allclose(out, want, rtol=1e-10, atol=1e-11)
+
+
+def test_derived_consts_of_builtin_models():
+    """IVIM reads -b (the exp factor, exp2-scaled) and b*b from derived rows,
+    T2 reads -TE; their host values, and the float32 exp2 form of the
+    emitted code against the float64 reference."""
+    from pycompute_amd import lm
+    dag = hipgen.res_jac_hes(IVIM.expr, IVIM.pars, IVIM.consts)
+    ent = []
+    hipgen.emit_model(dag, "X", "float32", reassoc=True, factored=True, derived=ent)
+    assert [(k, E.to_str(x)) for k, x in ent] == [("expk", "(-b)"), ("val", "(b*b)")]
+    b = np.linspace(0.0, 800.0, 16)
+    v = hipgen.derived_values(ent, dag, b[None, :], True)
+    np.testing.assert_allclose(v[0], -b / np.log(2.0), rtol=1e-15)
+    np.testing.assert_allclose(v[1], b * b, rtol=1e-15)
+    np.testing.assert_allclose(hipgen.derived_values(ent, dag, b[None, :], False)[0], -b)
+    t2 = MODELS["t2"]
+    d2 = hipgen.res_jac_hes(t2.expr, t2.pars, t2.consts)
+    e2 = []
+    hipgen.emit_model(d2, "Y", "float32", reassoc=True, factored=True, derived=e2)
+    assert [(k, E.to_str(x)) for k, x in e2] == [("expk", "(-TE)")]
+    # the module lists its derived rows; per-voxel and unfactored builds derive none
+    src_ent = []
+    spec = lm.LMSpec(IVIM.expr, IVIM.pars, IVIM.consts, 16, fast_exp=True, fast_div=True,
+                     reassoc=True, factored=True)
+    src = lm.build_source(spec, dag, (), src_ent)
+    assert len(src_ent) == 1 and len(src_ent[0]) == 2
+    assert "derived const c[1]" in src and "derived const c[2]" in src
+    for s in (lm.LMSpec(IVIM.expr, IVIM.pars, IVIM.consts, 16, fast_exp=True, reassoc=True,
+                        factored=True, consts_per_voxel=True),
+              lm.LMSpec(IVIM.expr, IVIM.pars, IVIM.consts, 16, fast_exp=True, reassoc=True)):
+        e = []
+        lm.build_source(s, dag, (), e)
+        assert e == [[]]
+    # a table that would not fit the consts block keeps the per-measurement form
+    big = []
+    lm.build_source(lm.LMSpec(IVIM.expr, IVIM.pars, IVIM.consts, 100, fast_exp=True,
+                              reassoc=True, factored=True), dag, (), big)
+    assert big == [[]]
+
+
+@pytest.mark.parametrize("name", ["ivim", "t2", "se"])
+def test_factored_derived_float32_exp2(name, tmp_path):
+    """Float32 build of the emitted code with exp as exp2 of the host-scaled
+    factor (the fast-exp form) against the float64 reference, at the bar
+    float32 sums of 16 measurements meet."""
+    m = MODELS[name]
+    dag = hipgen.res_jac_hes(m.expr, m.pars, m.consts)
+    L, ent = _build_factored_host(dag, "float32", True, tmp_path)
+    M = 16
+    cs = (np.linspace(50.0, 800.0, M) * (1e-3 if name != "t2" else 0.1))[None, :]
+    table = np.concatenate([cs, hipgen.derived_values(ent, dag, cs, True)]).astype(np.float32)
+    p = {"ivim": [1.0, 0.2, 3.0, 0.8], "t2": [1.0, 8.0], "se": [1.0, 1.2, 0.7]}[name]
+    p = np.array(p)
+    rng = np.random.default_rng(3)
+    f0, _, _, _ = _fghhl_ref(dag, p, list(cs), np.zeros(M), 0.1)
+    y = np.sqrt(f0 / M) * rng.uniform(0.5, 1.5, M)
+    f, g, h, hld = _fghhl_ref(dag, p, list(cs), y, 0.1)
+    want = np.concatenate([[f, f], g, h, hld])
+    fp = ctypes.POINTER(ctypes.c_float)
+    out = np.zeros(len(want), dtype=np.float32)
+    L.run_derived.argtypes = [fp, fp, fp, ctypes.c_int, ctypes.c_float, fp]
+    p32, y32 = p.astype(np.float32), y.astype(np.float32)
+    L.run_derived(p32.ctypes.data_as(fp), np.ascontiguousarray(table.ravel()).ctypes.data_as(fp),
+                  y32.ctypes.data_as(fp), M, 0.1, out.ctypes.data_as(fp))
+    np.testing.assert_allclose(out, want, rtol=2e-4, atol=2e-4 * np.max(np.abs(want)))
+
+
 @pytest.mark.parametrize("name,M", [("ivim", 16), ("t2", 32), ("se", 16)])
 def test_hiprtc_compiles_factored(name, M):
     from pycompute_amd import lm

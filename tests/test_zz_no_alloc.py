@@ -19,7 +19,7 @@ KNOBS = dict(fast_exp=True, fast_div=True, reassoc=True, factored=True, chunk=8)
 def _static_fit(prob, p0, iters, chunk):
     s = fit.make_solver(prob, compaction=True, keep_workspace=False, **KNOBS)
     s.setup(p0.clone(), prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
-    uc = torch.tensor(s._uc, dtype=torch.float32, device=prob.data_t.device)
+    uc = torch.tensor(s._ucx, dtype=torch.float32, device=prob.data_t.device)
     args = (_lib.LMArgs * 1)(s.lm_args(iters, 1e-8, 0.25, 0.75, 0.2, 2.0))
     ucp = (ctypes.c_void_p * 1)(uc.data_ptr())
     npar = (ctypes.c_int32 * 1)(4)
