@@ -179,39 +179,16 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
     MLP-initialised fit of the benchmark volume and check the residual match on
     it."""
     import torch
-    from pycompute_amd import fit, mlp, synth
+    from pycompute_amd import fit, synth
 
     N = prob.N
-    net = mlp.ivim_mlp(synth.B_C2, hidden=args.mlp_hidden, seed=0)
-    t0 = time.perf_counter()
-    data, truth = net.synthesize(args.train_samples, seed=20240613, snr=(20.0, 100.0),
-                                 lo=IVIM_BOX[0], hi=IVIM_BOX[1])
-    net.fit_standardization(truth, data)
-    X, T = net.prepare(data, truth)
-    del data, truth
-    hist = net.train(X, T, epochs=args.train_epochs, batch=8192, log=log)
-    torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    del X, T
-    log(f"mlp trained on {args.train_samples} signals in {train_s:.2f} s, loss {hist}")
-
-    def mlp_guess(p, pr):
-        # voxels with non-finite outputs get the default guess in the same kernel
-        net.infer(pr.data_t, pr.lb_t, pr.ub_t, p, fallback=(pr.model, pr.consts))
-
+    net, train_s, hist = _train_ivim_net(args, args.mlp_hidden)
     # validation volume: choose the iteration cap
     wv = synth.ivim(1 << 18, synth.B_C2, seed=99, snr=(20.0, 100.0))
     pv = fit.make_problem("ivim", wv["consts"], wv["data"], device=prob.data_t.device)
     ref = fit.fit(pv, iters=100, **knobs)
     ref_ssr = float(ref.final_f().double().sum())
-    pguess = torch.empty((4, pv.N), dtype=torch.float32, device=pv.data_t.device)
-    mlp_guess(pguess, pv)
-    cap = ITER_CAPS[-1]
-    for c in ITER_CAPS:
-        s = fit.fit(pv, iters=c, params0=pguess.clone(), **knobs)
-        if float(s.final_f().double().sum()) <= ref_ssr * (1 + 1e-4):
-            cap = c
-            break
+    cap = _validated_cap(net, pv, ref_ssr, knobs)
     log(f"validation: iteration cap {cap}")
     # what capping alone buys: the default guess at its own validated cap
     pdef = fit.default_init(pv)
@@ -301,6 +278,7 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
     log(f"mlp-init fit {ms:.3f} ms/step, cap {cap}, {it:.2f} iterations/voxel, total SSR "
         f"{ssr:.4f} vs plain {plain_ssr:.4f}; inference {infer_ms:.3f} ms")
     return {"value": N / (ms * 1e-3), "ms_per_step": ms, "iter_cap": cap,
+            "network": f"16-{net.H}-{net.H}-4 GELU",
             "iterations_per_voxel": it, "total_final_ssr": ssr,
             "total_ssr_rel_vs_plain": ssr / plain_ssr - 1.0,
             "matched_total_residual": bool(ssr <= plain_ssr * (1 + 1e-4)),
@@ -317,7 +295,43 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
             "training_break_even_volumes": (train_s / ((plain_ms - ms) * 1e-3)
                                             if plain_ms and plain_ms > ms else None),
             **extra,
-            "_net": net, "_cap": cap}
+            "_net": net, "_cap": cap, "_val": (pv, ref_ssr)}
+
+
+def _train_ivim_net(args, hidden):
+    """The IVIM initialiser of width ``hidden``, trained on args.train_samples
+    forward-model signals (truth targets, seed 20240613); returns (net,
+    seconds, loss history)."""
+    import torch
+    from pycompute_amd import mlp, synth
+    net = mlp.ivim_mlp(synth.B_C2, hidden=hidden, seed=0)
+    t0 = time.perf_counter()
+    data, truth = net.synthesize(args.train_samples, seed=20240613, snr=(20.0, 100.0),
+                                 lo=IVIM_BOX[0], hi=IVIM_BOX[1])
+    net.fit_standardization(truth, data)
+    X, T = net.prepare(data, truth)
+    del data, truth
+    hist = net.train(X, T, epochs=args.train_epochs, batch=8192, log=log)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"mlp (width {hidden}) trained on {args.train_samples} signals in {train_s:.2f} s, "
+        f"loss {hist}")
+    return net, train_s, hist
+
+
+def _validated_cap(net, pv, ref_ssr, knobs):
+    """The smallest LM iteration cap from the network's guess whose total final
+    SSR on the validation problem pv is within 1e-4 of plain LM's (ref_ssr)."""
+    import torch
+    from pycompute_amd import fit
+    g = torch.empty((4, pv.N), dtype=torch.float32, device=pv.data_t.device)
+    # voxels with non-finite outputs get the default guess in the same kernel
+    net.infer(pv.data_t, pv.lb_t, pv.ub_t, g, fallback=(pv.model, pv.consts))
+    for c in ITER_CAPS:
+        s = fit.fit(pv, iters=c, params0=g.clone(), **knobs)
+        if float(s.final_f().double().sum()) <= ref_ssr * (1 + 1e-4):
+            return c
+    return ITER_CAPS[-1]
 
 
 def run_mixed(args, stream):
@@ -439,18 +453,26 @@ def run_mixed(args, stream):
             "per_model": per_model}
 
 
-def run_ivim_4m(args, knobs, net, cap):
+def run_ivim_4m(args, knobs, net, cap, val):
     """The 4,194,304-voxel IVIM volume (16 b-values, SNR U[20, 100], seed 4):
     plain LM (default guess, at most 100 iterations) and the MLP-initialised
-    fit at the cap validated on the 1M run, both as captured graphs."""
+    fit, both as captured graphs. The network is the configuration's
+    16-128-128-4 GELU MLP: the 1M run's own network when it has that width,
+    else one trained the same way, its cap validated on the same volume."""
     import torch
     from pycompute_amd import fit, synth
 
+    train_s = None
+    if net.H != 128:
+        net, train_s, _ = _train_ivim_net(args, 128)
+        cap = _validated_cap(net, val[0], val[1], knobs)
+        log(f"4M network (width 128): validated cap {cap}")
     w = synth.ivim(1 << 22, synth.B_C2, seed=4, snr=(20.0, 100.0))
     prob = fit.make_problem("ivim", w["consts"], w["data"])
     N = prob.N
     out = {"workload": "IVIM 4,194,304 voxels, 16 b-values 0-800, SNR U[20,100]",
-           "voxels": N}
+           "voxels": N, "network": f"16-{net.H}-{net.H}-4 GELU", "iter_cap": cap,
+           "train_seconds": train_s}
     finals = {}
     ns = 8192
     for name, iters, use_mlp in (("plain_lm", 100, False), ("mlp_lm", cap, True)):
@@ -683,7 +705,8 @@ def main():
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--mlp-hidden", type=int, default=64, choices=(32, 64, 128, 256),
-                    help="hidden width of the IVIM initialiser")
+                    help="hidden width of the 1M-voxel fit's IVIM initialiser (the "
+                         "4M-voxel volume uses the configuration's width 128)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-model batch")
     ap.add_argument("--tuning", type=str, default=None,
                     help="JSON knob set for the IVIM fit instead of the autotuner's choice")
@@ -818,8 +841,9 @@ def main():
         mlp_info = run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=plain_f,
                            plain_ms=ms_step, w=w)
         value, ms_value = mlp_info.pop("value"), mlp_info.pop("ms_per_step")
-    net, cap = (mlp_info.pop("_net"), mlp_info.pop("_cap")) if mlp_info else (None, None)
-    ivim_4m = None if (args.no_4m or net is None) else run_ivim_4m(args, knobs, net, cap)
+    net, cap, val = ((mlp_info.pop("_net"), mlp_info.pop("_cap"), mlp_info.pop("_val"))
+                     if mlp_info else (None, None, None))
+    ivim_4m = None if (args.no_4m or net is None) else run_ivim_4m(args, knobs, net, cap, val)
     del net
     mixed = None if args.no_mixed else run_mixed(args, stream)
     dki = None if args.no_dki else run_dki(args, stream)
