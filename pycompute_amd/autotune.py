@@ -19,6 +19,11 @@ of the caller's own problem:
   * vpl:           voxels per lane (1 or 2; 64 * vpl voxels per wavefront):
                    two voxels in lockstep interleave two independent chains
                    at twice the registers per lane
+  * exp_rec:       exponential recurrence (factored builds): exp(k_i x) directly
+                   every exp_rec measurements, one multiply in between; only
+                   for protocols whose exponential factor rows are arithmetic
+                   progressions (other problems skip these candidates)
+  * uc_lds:        the consts tables read from LDS instead of the vector cache
   * waves_per_eu:  occupancy hint (mixed-model batches, batch_candidates())
   * tile:          wide-model kernel register/LDS split (wide_candidates())
   * mfma:          generalised linear models with n > 8: the Hessian on the
@@ -41,7 +46,7 @@ from . import program
 
 DEFAULT = {"block": 256, "chunk": 16, "max_chunks": 0, "unroll": 0, "fast_exp": False,
            "fast_div": False, "reassoc": False, "waves_per_eu": 0, "factored": False,
-           "sched_group": 0, "vpl": 1}
+           "sched_group": 0, "vpl": 1, "exp_rec": 0, "uc_lds": False}
 
 
 def _cache_path() -> str:
@@ -82,18 +87,23 @@ def candidates(nparam: int, allow_fast: bool):
                  (True, True, True, 0, False, 0), (True, True, True, 2, False, 0),
                  (True, True, True, 2, True, 0), (True, True, True, 0, True, 4)]
     for fe, fd, ra, wpe, fac, sg in fast:
-        for chunk, mc in ((8, 0), (16, 0), (24, 0), (16, 4)):
-            for block in (256, 128):
-                out.append({"block": block, "chunk": chunk, "max_chunks": mc, "unroll": 0,
-                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                            "waves_per_eu": wpe, "factored": fac, "sched_group": sg,
-                            "vpl": 1})
+        # the factored builds also with the exponential recurrence (anchors
+        # every 4 or 8 measurements) and the consts tables in LDS
+        recs = ((0, False), (4, True), (8, True)) if fac and nparam <= 8 else ((0, False),)
+        for rec, lds in recs:
+            for chunk, mc in ((8, 0), (16, 0), (24, 0), (16, 4)):
+                for block in (256, 128):
+                    out.append({"block": block, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                                "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                                "waves_per_eu": wpe, "factored": fac, "sched_group": sg,
+                                "vpl": 1, "exp_rec": rec, "uc_lds": lds})
         if sg == 0 and nparam <= 8:
             # two voxels per lane (one wavefront per SIMD for most models)
             for chunk, mc in ((16, 0), (16, 4)):
                 c = {"block": 128, "chunk": chunk, "max_chunks": mc, "unroll": 0,
                      "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                     "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2}
+                     "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2,
+                     "exp_rec": 0, "uc_lds": False}
                 if c not in out:
                     out.append(c)
     return out
@@ -113,17 +123,20 @@ def batch_candidates(allow_fast: bool = True):
                 out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
                             "fast_exp": fe, "fast_div": fd, "reassoc": ra,
                             "waves_per_eu": wpe, "factored": fac, "sched_group": 0,
-                            "vpl": 1})
+                            "vpl": 1, "exp_rec": 0, "uc_lds": False})
         out.append({"block": 128, "chunk": 16, "max_chunks": 4, "unroll": 0,
                     "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                    "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2})
+                    "waves_per_eu": 0, "factored": fac, "sched_group": 0, "vpl": 2,
+                    "exp_rec": 0, "uc_lds": False})
         if fac:
             # a register fence every 4 measurements: 3 waves per SIMD
             # instead of 2 for the three-model batch
             for chunk, mc in ((16, 0), (16, 4)):
-                out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
-                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                            "waves_per_eu": 0, "factored": fac, "sched_group": 4, "vpl": 1})
+                for rec, lds in ((0, False), (4, True)):
+                    out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                                "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                                "waves_per_eu": 0, "factored": fac, "sched_group": 4,
+                                "vpl": 1, "exp_rec": rec, "uc_lds": lds})
     return out
 
 
@@ -177,11 +190,25 @@ def tune(make_solver, setup, model_expr: str, M: int, dtype: str, allow_fast: bo
     k = key(model_expr, M, dtype, allow_fast, arch)
     db = _load()
     if not force and k in db:
-        return dict(db[k]["best"])
+        best = dict(db[k]["best"])
+        try:
+            # a knob set that does not apply to this problem's protocol (the
+            # exponential recurrence) is tuned again
+            make_solver(compaction=True, **best)
+            return best
+        except ValueError:
+            pass
     results = []
     ref_f = None
     for cand in (cands or candidates(0, allow_fast)):
-        s = make_solver(compaction=True, **cand)
+        try:
+            s = make_solver(compaction=True, **cand)
+        except ValueError as err:
+            if ref_f is None:
+                raise
+            if log:
+                log(f"autotune {cand} skipped: {err}")
+            continue
         ts = []
         for r in range(reps + 1):
             setup(s)

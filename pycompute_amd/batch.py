@@ -75,7 +75,8 @@ class BatchedLM:
                  fast_div: bool = False,
                  strict: bool = False, unroll: int = 0, waves_per_eu: int = 0,
                  reassoc: bool = False, sched_group: int = 0, factored: bool = False,
-                 vpl: int = 1, compaction: bool = True, **_):
+                 vpl: int = 1, compaction: bool = True, exp_rec: int = 0,
+                 uc_lds: bool = False, **_):
         if not 1 <= len(models) <= 4:
             raise ValueError("a batch holds 1 to 4 models")
         self.dtype_name = lm.dtype_name(dtype)
@@ -89,7 +90,8 @@ class BatchedLM:
                                    bool(strict), unroll=int(unroll),
                                    waves_per_eu=int(waves_per_eu), fast_div=bool(fast_div),
                                    reassoc=bool(reassoc), sched_group=int(sched_group),
-                                   factored=bool(factored), vpl=int(vpl)))
+                                   factored=bool(factored), vpl=int(vpl),
+                                   exp_rec=int(exp_rec), uc_lds=bool(uc_lds)))
         self.members = [Member(s) for s in specs]
         self.program = lm.get_program(specs[0], tuple(specs[1:]))
         self.funcid = self.program.funcid

@@ -56,6 +56,12 @@
 #ifndef PCA_FACTORED
 #define PCA_FACTORED 0
 #endif
+// Exponential recurrence (PCA_EXP_REC = R > 0, factored builds whose
+// exponential factor rows are arithmetic progressions over the measurements):
+// exp(k_i x) is evaluated directly every R measurements and by one multiply
+// in between (pca_exp_rec in the prelude); the host checks the rows. The
+// default PCA_EXP_REC 0 is set in the prelude.
+//
 // Voxels per lane of the chunked kernel (1 or 2): a lane iterates PCA_VPL
 // queue entries in lockstep, so their independent dependency chains
 // interleave (64 * PCA_VPL voxels per wavefront).
@@ -103,7 +109,7 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
     // the same sums with the parameter-only factors applied after the loop
     typename Mdl::Pre P;
     Mdl::prep(V.p, V.lam, P);
-    real s[Mdl::NS], q[NH], dmp[NP];
+    real s[Mdl::NS], q[NH], dmp[NP], ex[Mdl::NE], qx[Mdl::NE];
 #pragma unroll
     for (int k = 0; k < Mdl::NS; ++k) s[k] = 0;
 #pragma unroll
@@ -115,7 +121,7 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
       real c[Mdl::NCT];
 #pragma unroll
       for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(k, i);
-      Mdl::accf(P, c, V.y[i], V.f, s, q, dmp);
+      Mdl::accf(P, c, V.y[i], V.f, s, q, dmp, i, ex, qx);
       if constexpr (PCA_SCHED_GROUP > 0) {
         if (i % PCA_SCHED_GROUP == PCA_SCHED_GROUP - 1 && i + 1 < M) {
           reg_fence(V.f);
@@ -124,6 +130,10 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
           reg_fence(dmp);
           reg_fence(P.a);
           reg_fence(P.L);
+          if constexpr (Mdl::EXP_REC) {
+            reg_fence(ex);
+            reg_fence(qx);
+          }
         }
       }
     }
@@ -192,17 +202,22 @@ __device__ __forceinline__ void iterate(Voxel<Mdl, M>& V, const Settings& S, con
   if constexpr (factored) {
     typename Mdl::RPre R;
     Mdl::rprep(ptp, R);
+    real ex[Mdl::NE], qx[Mdl::NE];
 #pragma unroll UNROLL
     for (int i = 0; i < M; ++i) {
       real c[Mdl::NCT];
 #pragma unroll
       for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(k, i);
-      const real r = Mdl::residf(R, c, V.y[i]);
+      const real r = Mdl::residf(R, c, V.y[i], i, ex, qx);
       ftp += r * r;
       if constexpr (PCA_SCHED_GROUP > 0) {
         if (i % PCA_SCHED_GROUP == PCA_SCHED_GROUP - 1 && i + 1 < M) {
           reg_fence(ftp);
           reg_fence(R.a);
+          if constexpr (Mdl::EXP_REC) {
+            reg_fence(ex);
+            reg_fence(qx);
+          }
         }
       }
     }
@@ -282,7 +297,7 @@ __device__ __forceinline__ void iterate_n(Voxel<Mdl, M> (&V)[VPL], const Setting
   // fghhl (lsqnonlin.py:206-346)
   if constexpr (factored) {
     typename Mdl::Pre P[VPL];
-    real s[VPL][Mdl::NS], q[VPL][NH], dmp[VPL][NP];
+    real s[VPL][Mdl::NS], q[VPL][NH], dmp[VPL][NP], ex[VPL][Mdl::NE], qx[VPL][Mdl::NE];
 #pragma unroll
     for (int u = 0; u < VPL; ++u) {
       Mdl::prep(V[u].p, V[u].lam, P[u]);
@@ -300,7 +315,7 @@ __device__ __forceinline__ void iterate_n(Voxel<Mdl, M> (&V)[VPL], const Setting
         real c[Mdl::NCT];
 #pragma unroll
         for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(u, k, i);
-        Mdl::accf(P[u], c, V[u].y[i], f[u], s[u], q[u], dmp[u]);
+        Mdl::accf(P[u], c, V[u].y[i], f[u], s[u], q[u], dmp[u], i, ex[u], qx[u]);
       }
     }
 #pragma unroll
@@ -367,6 +382,7 @@ __device__ __forceinline__ void iterate_n(Voxel<Mdl, M> (&V)[VPL], const Setting
   }
   if constexpr (factored) {
     typename Mdl::RPre R[VPL];
+    real ex[VPL][Mdl::NE], qx[VPL][Mdl::NE];
 #pragma unroll
     for (int u = 0; u < VPL; ++u) Mdl::rprep(ptp[u], R[u]);
 #pragma unroll UNROLL
@@ -376,7 +392,7 @@ __device__ __forceinline__ void iterate_n(Voxel<Mdl, M> (&V)[VPL], const Setting
         real c[Mdl::NCT];
 #pragma unroll
         for (int k = 0; k < Mdl::NCT; ++k) c[k] = cf(u, k, i);
-        const real r = Mdl::residf(R[u], c, V[u].y[i]);
+        const real r = Mdl::residf(R[u], c, V[u].y[i], i, ex[u], qx[u]);
         ftp[u] += r * r;
       }
     }
@@ -577,6 +593,13 @@ struct pca_chunk_args {
                          // reading it: a captured fit needs no reset kernel
 };
 
+#if defined(PCA_UC_LDS) && !defined(PCA_CONSTS_PER_VOXEL)
+// The voxel-uniform consts tables of every model, copied to LDS when the
+// chunk kernel starts (PCA_UC_LDS): the measurement loop reads them at LDS
+// latency instead of the vector cache's.
+__shared__ real pca_uc_lds[PCA_NMODELS][PCA_UC_MAX];
+#endif
+
 namespace pca {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
@@ -681,6 +704,8 @@ __device__ __forceinline__ void chunk_wave(const pca_chunk_args& A, long long e0
   for (int u = 0; u < VPL; ++u) cnt[u] = 0;
 #ifdef PCA_CONSTS_PER_VOXEL
   auto cfu = [&](int u, int k, int i) { return cv[u][(k * M + i) % NCV]; };
+#elif defined(PCA_UC_LDS)
+  auto cfu = [&](int u, int k, int i) { return pca_uc_lds[MI][k * M + i]; };
 #else
   const real* __restrict__ uc = J.uc;
   auto cfu = [&](int u, int k, int i) { return uc[k * M + i]; };
@@ -780,6 +805,20 @@ PCA_MODEL_AT(2)
 PCA_MODEL_AT(3)
 #endif
 
+#if defined(PCA_UC_LDS) && !defined(PCA_CONSTS_PER_VOXEL)
+// copies model MI's (and the following models') consts tables into LDS; the
+// caller synchronises the workgroup afterwards
+template <int MI>
+__device__ __forceinline__ void load_uc_lds(const pca_chunk_args& A) {
+  if constexpr (MI < PCA_NMODELS) {
+    constexpr int n = ModelAt<MI>::type::NCT * ModelAt<MI>::M;
+    const real* __restrict__ uc = A.job[MI].uc;
+    for (int q = threadIdx.x; q < n; q += PCA_BLOCK) pca_uc_lds[MI][q] = uc[q];
+    load_uc_lds<MI + 1>(A);
+  }
+}
+#endif
+
 template <int MI>
 __device__ __forceinline__ void dispatch(int m, const pca_chunk_args& A, long long e0,
                                          long long count) {
@@ -811,6 +850,10 @@ pca_lm_chunk(pca_chunk_args A) {
   for (int m = 0; m < PCA_NMODELS; ++m)
     tasks[m + 1] = tasks[m] + ((long long)cnt[m] + 64 * PCA_VPL - 1) / (64 * PCA_VPL);
   const long long total = tasks[PCA_NMODELS];
+#if defined(PCA_UC_LDS) && !defined(PCA_CONSTS_PER_VOXEL)
+  pca::load_uc_lds<0>(A);
+  __syncthreads();
+#endif
   const long long waves_per_block = PCA_BLOCK / 64;
   const long long nwaves = (long long)gridDim.x * waves_per_block;
   for (long long t = (long long)blockIdx.x * waves_per_block + threadIdx.x / 64; t < total;

@@ -86,14 +86,47 @@ __device__ __forceinline__ float pca_rcp_fast(float a) { return __builtin_amdgcn
 __device__ __forceinline__ double pca_rcp_fast(double a) { return 1.0 / a; }
 __device__ __forceinline__ float pca_sqrt_fast(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double pca_sqrt_fast(double a) { return sqrt(a); }
-// max(L*uu, 1e-4) - L*uu = max(1e-4 - L*uu, 0): the damping floor's excess
-// over L*uu (factored accumulation). For float one FMA with the clamp
-// modifier: the value lies below 1e-4, so clamping to [0, 1] is the max.
-__device__ __forceinline__ float pca_floor_excess(float L, float uu) {
-  return __builtin_amdgcn_fmed3f(__builtin_fmaf(-L, uu, PCA_DAMP_FLOOR), 0.0f, 1.0f);
+// A wave-uniform value held in a VGPR. On gfx950 a VALU instruction that
+// reads an SGPR operand issues at about half the rate of its all-VGPR (or
+// literal) form, so constants that a hot loop reads many times are pinned
+// to a VGPR once (the empty asm makes the value opaque, hence per-lane).
+__device__ __forceinline__ float pca_vgpr(float x) {
+  asm volatile("" : "+v"(x));
+  return x;
 }
-__device__ __forceinline__ double pca_floor_excess(double L, double uu) {
-  return fmax(__builtin_fma(-L, uu, PCA_DAMP_FLOOR), 0.0);
+__device__ __forceinline__ double pca_vgpr(double x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+// max(L*uu, fl) - L*uu = max(fl - L*uu, 0), fl the damping floor 1e-4: the
+// floor's excess over L*uu (factored accumulation). For float one FMA with
+// the clamp modifier: the value lies below 1e-4, so clamping to [0, 1] is the
+// max. fl comes from a VGPR (pca_vgpr), keeping the FMA at the full rate.
+__device__ __forceinline__ float pca_floor_excess(float L, float uu, float fl) {
+  return __builtin_amdgcn_fmed3f(__builtin_fmaf(-L, uu, fl), 0.0f, 1.0f);
+}
+__device__ __forceinline__ double pca_floor_excess(double L, double uu, double fl) {
+  return fmax(__builtin_fma(-L, uu, fl), 0.0);
+}
+// exp(k_i x) at measurement i of an exponential whose factor row k is an
+// arithmetic progression, k_i = k_0 + i d (exponential recurrence, factored
+// builds with PCA_EXP_REC = R > 0): every R-th measurement evaluates it
+// directly, the ones between multiply the previous value by q = exp(d x),
+// which measurement 0 evaluates. ks = d (scaled like k). With the measurement
+// loop unrolled, i is a constant and the branches fold away. Relative error:
+// at most R - 1 rounded products and the error of q times R - 1.
+#ifndef PCA_EXP_REC
+#define PCA_EXP_REC 0
+#endif
+template <int R>
+__device__ __forceinline__ real pca_exp_rec(int i, real& e, real& q, real k, real ks, real x) {
+  static_assert(R > 0, "the exponential recurrence needs PCA_EXP_REC > 0");
+  if (i == 0) q = PCA_EXPKD(ks, x);
+  if (i % R == 0)
+    e = PCA_EXPKD(k, x);
+  else
+    e = e * q;
+  return e;
 }
 __device__ __forceinline__ bool pca_nonfinite(float a) { return __builtin_isnan(a) || __builtin_isinf(a); }
 __device__ __forceinline__ bool pca_nonfinite(double a) { return __builtin_isnan(a) || __builtin_isinf(a); }

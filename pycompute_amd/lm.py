@@ -69,6 +69,13 @@ class LMSpec:
     # voxels per lane of the chunked kernel (1 or 2; 64 * vpl voxels per
     # wavefront): a lane iterates vpl voxels in lockstep
     vpl: int = 1
+    # exponential recurrence (factored builds with voxel-uniform consts whose
+    # exponential factor rows are arithmetic progressions): exp(k_i x)
+    # directly every exp_rec measurements, one multiply in between (0 = off)
+    exp_rec: int = 0
+    # the chunk kernel copies the voxel-uniform consts tables to LDS and reads
+    # them there (lower latency than the vector cache)
+    uc_lds: bool = False
 
     @property
     def nparam(self):
@@ -122,6 +129,13 @@ class LMProgram:
             raise ValueError(f"vpl must be 1 or 2, got {spec.vpl}")
         if spec.vpl > 1 and spec.sched_group > 0:
             raise ValueError("register fences (sched_group) apply to vpl = 1 only")
+        if spec.exp_rec < 0:
+            raise ValueError(f"exp_rec must be >= 0, got {spec.exp_rec}")
+        if spec.exp_rec > 0:
+            for s_ in (spec,) + tuple(members):
+                if not s_.factored or s_.consts_per_voxel or _unroll(s_) != s_.M:
+                    raise ValueError("exp_rec needs the factored build with voxel-uniform "
+                                     "consts and a fully unrolled measurement loop")
         options = list(BASE_OPTIONS)
         if spec.strict:
             options.append("-ffp-contract=off")
@@ -209,6 +223,10 @@ def build_source(spec: LMSpec, dag, members=(), derived: list | None = None) -> 
         head.append(f"#define PCA_SCHED_GROUP {spec.sched_group}")
     if spec.vpl > 1:
         head.append(f"#define PCA_VPL {spec.vpl}")
+    if spec.exp_rec > 0:
+        head.append(f"#define PCA_EXP_REC {spec.exp_rec}")
+    if spec.uc_lds:
+        head.append("#define PCA_UC_LDS 1")
     structs, typedefs, entries = {}, [], {}
     maxM: dict = {}
     for s, _ in models:
@@ -219,10 +237,14 @@ def build_source(spec: LMSpec, dag, members=(), derived: list | None = None) -> 
         if sname not in structs:
             ent = [] if derive_consts(spec) else None
             txt = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc,
-                                    factored=spec.factored, derived=ent)
+                                    factored=spec.factored, derived=ent,
+                                    exp_rec=spec.exp_rec > 0)
             if ent and (d.nconst + len(ent)) * maxM[sname] > _lib.PCA_UC_MAX:
                 # the derived rows would not fit the consts block: computed
                 # per measurement instead
+                if spec.exp_rec > 0:
+                    raise ValueError("exp_rec: the derived consts rows exceed the "
+                                     f"{_lib.PCA_UC_MAX}-value consts block")
                 ent = None
                 txt = hipgen.emit_model(d, sname, spec.dtype, reassoc=spec.reassoc,
                                         factored=spec.factored)

@@ -144,7 +144,8 @@ class SecondOrderLevenbergMarquardt(HipFunction):
                  compaction: bool = True, chunk: int = 8, max_chunks: int = 0, unroll: int = 0,
                  waves_per_eu: int = 0, fast_div: bool = False, wide: bool | None = None,
                  tile: int = 136, reassoc: bool = False, sched_group: int = 0,
-                 factored: bool = False, vpl: int = 1, mfma: bool = False, **kwargs):
+                 factored: bool = False, vpl: int = 1, mfma: bool = False, exp_rec: int = 0,
+                 uc_lds: bool = False, **kwargs):
         if (args and isinstance(args[0], str)) or "expr" in kwargs:
             # notebook form: (expr, pars_str, consts_str, ndata, dtype, write_to_file)
             a = _bind(args, kwargs, ("expr", "pars_str", "consts_str", "ndata", "dtype",
@@ -197,7 +198,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         self._knobs = dict(unroll=int(unroll), waves_per_eu=int(waves_per_eu),
                            fast_div=bool(fast_div), reassoc=bool(reassoc),
                            sched_group=int(sched_group), factored=bool(factored),
-                           vpl=int(vpl))
+                           vpl=int(vpl), exp_rec=int(exp_rec), uc_lds=bool(uc_lds))
         self.keep_workspace = (self.nparam <= 8) if keep_workspace is None else bool(keep_workspace)
         # wide: the kernels for models with more parameters than one lane's
         # registers hold (n > 8): csrc/lm_glm.hip for generalised linear

@@ -165,7 +165,8 @@ class SecondOrderRandomSearch(_SearchBase):
                  strict: bool = False, batch: int | None = None, **kwargs):
         lm_kw = {k: kwargs.pop(k) for k in list(kwargs)
                  if k in ("compaction", "chunk", "max_chunks", "fast_exp", "fast_div", "reassoc",
-                          "unroll", "waves_per_eu", "sched_group", "factored", "vpl", "wide")}
+                          "unroll", "waves_per_eu", "sched_group", "factored", "vpl", "wide",
+                          "exp_rec", "uc_lds")}
         super().__init__(*args, seed=seed, block=block, strict=strict, **kwargs)
         if batch is not None and int(batch) < 1:
             raise ValueError("batch must be at least 1")

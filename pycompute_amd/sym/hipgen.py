@@ -254,7 +254,10 @@ class DerivedConsts:
 
     entries[i] = (kind, node): kind "val" is the node's value; kind "expk"
     is the factor k of an exp(k*x), scaled by log2(e) when the fast-exp
-    float build evaluates exp through exp2 (PCA_EXPKD)."""
+    float build evaluates exp through exp2 (PCA_EXPKD); kind "expstep" is the
+    common difference d of the "expk" row of the same node, which must be an
+    arithmetic progression k_i = k_0 + i d over the measurements (the
+    exponential recurrence, pca_exp_rec), in every column of its row."""
 
     def __init__(self, dag: ModelDAG, fac: "_Factorer"):
         self.nc = dag.nconst
@@ -287,8 +290,34 @@ def derived_values(entries, dag: ModelDAG, consts_rows, exp2_scale: bool):
     out = np.empty((len(entries), M))
     for i, (kind, n) in enumerate(entries):
         v = np.broadcast_to(np.asarray(E.evaluate(n, env, memo), dtype=np.float64), (M,))
-        out[i] = v * (1.0 / np.log(2.0) if kind == "expk" and exp2_scale else 1.0)
+        v = v * (1.0 / np.log(2.0) if kind in ("expk", "expstep") and exp2_scale else 1.0)
+        if kind == "expstep":
+            v = np.full(M, arithmetic_step(v, E.to_str(n)))
+        out[i] = v
     return out
+
+
+# Largest deviation of an exponential-recurrence row from its arithmetic
+# progression, relative to the row's largest magnitude: the rounding of float32
+# consts (half an ulp each) passes, anything else does not.
+EXP_REC_TOL = 2.0 ** -22
+
+
+def arithmetic_step(v, what: str = "row") -> float:
+    """The common difference d of a row v_i = v_0 + i d (float64), or
+    ValueError when the row is not an arithmetic progression to within
+    EXP_REC_TOL of its largest magnitude."""
+    v = np.asarray(v, dtype=np.float64)
+    M = len(v)
+    if M < 2:
+        return 0.0
+    d = (v[-1] - v[0]) / (M - 1)
+    dev = np.max(np.abs(v - (v[0] + d * np.arange(M))))
+    if not np.all(np.isfinite(v)) or dev > EXP_REC_TOL * np.max(np.abs(v)):
+        raise ValueError(f"the exponential recurrence needs the consts factor {what} to be "
+                         f"an arithmetic progression over the measurements "
+                         f"(largest deviation {dev:.3g})")
+    return float(d)
 
 
 class _MeasPrinter(_Printer):
@@ -299,13 +328,17 @@ class _MeasPrinter(_Printer):
     DerivedConsts registry, const-only subexpressions are read from the
     consts table instead and exp(k*x) becomes PCA_EXPKD(scaled k, x)."""
 
-    def __init__(self, dag, dtype, fac: _Factorer, pre: str, derived: DerivedConsts | None = None):
+    def __init__(self, dag, dtype, fac: _Factorer, pre: str, derived: DerivedConsts | None = None,
+                 exp_rec: bool = False):
         super().__init__(dag, dtype)
         self.fac, self.pre_name = fac, pre
         self.pre: dict = {}
         self.pre_nodes: list = []
         self._expk: dict = {}
         self.derived = derived
+        # exponential recurrence: one running value per distinct exp(k*x)
+        self.exp_rec = exp_rec and derived is not None
+        self.rec_terms: dict = {}
 
     def _hoisted(self, n: E.Node) -> bool:
         return self.derived is not None and self.derived.hoists(n)
@@ -348,7 +381,12 @@ class _MeasPrinter(_Printer):
                         continue
                     if node.id in self._expk:
                         k, x = self._expk[node.id]
-                        if self.derived is not None:
+                        if self.exp_rec:
+                            e = self.rec_terms.setdefault((k.id, x.id), len(self.rec_terms))
+                            txt = (f"pca_exp_rec<PCA_EXP_REC>(i, ex[{e}], qx[{e}], "
+                                   f"{self.derived.ref('expk', k)}, "
+                                   f"{self.derived.ref('expstep', k)}, {self.ref(x)})")
+                        elif self.derived is not None:
                             txt = f"PCA_EXPKD({self.derived.ref('expk', k)}, {self.ref(x)})"
                         else:
                             txt = f"PCA_EXPK({self.ref(k)}, {self.ref(x)})"
@@ -370,7 +408,7 @@ class _MeasPrinter(_Printer):
                         stack.append((c, False))
 
 
-def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
+def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False, exp_rec: bool = False):
     """Members of the factored accumulation (fast builds), and the
     DerivedConsts entries they read (empty unless ``derive``).
 
@@ -388,7 +426,12 @@ def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
     u_j^2 is the Gauss-Newton sum already accumulated for h_jj.
     Sums over the same u are shared (a second derivative often has the u of a
     Jacobian entry). The model m = A_m u_m gives r = fma(A_m, u_m, -y). Equal
-    to accum() up to rounding."""
+    to accum() up to rounding.
+
+    accf and residf take the measurement index i and the running values ex,
+    qx of the exponential recurrence (``exp_rec``, with ``derive``): each
+    exp(k*x) whose factor row k is an arithmetic progression is evaluated
+    through pca_exp_rec (NE terms); without it they are unused."""
     n, nh = dag.nparam, dag.nhes
     fac = _Factorer(dag)
     g = dag.graph
@@ -410,7 +453,7 @@ def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
 
     # accf: one measurement
     dc = DerivedConsts(dag, fac) if derive else None
-    pm = _MeasPrinter(dag, dtype, fac, "P", dc)
+    pm = _MeasPrinter(dag, dtype, fac, "P", dc, exp_rec)
     pm.emit([mu] + [u for _, u in jf] + [x[1] for x in hf if x is not None])
     acc = list(pm.lines)
     acc.append(f"const real r = {resid(pm)};")
@@ -423,7 +466,7 @@ def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
         for k in range(j + 1):
             if j == k:
                 acc.append(f"{{ const real uu = {uj}*{uj}; q[{l}] += uu; "
-                           f"dmp[{j}] += pca_floor_excess(P.L[{j}], uu); }}")
+                           f"dmp[{j}] += pca_floor_excess(P.L[{j}], uu, P.fl); }}")
             else:
                 acc.append(f"q[{l}] = pca_fma({uj}, {pm.ref(jf[k][1])}, q[{l}]);")
             l += 1
@@ -435,6 +478,7 @@ def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
     prep += [f"P.a[{i}] = {pp.ref(x)};" for i, x in enumerate(pm.pre_nodes)]
     prep += [f"{{ const real a = {pp.ref(A)}; P.L[{j}] = lam*(a*a); }}"
              for j, (A, _) in enumerate(jf)]
+    prep.append(f"P.fl = pca_vgpr((real){floor});")
 
     # finish: apply the factors
     pf = _Printer(dag, dtype)
@@ -457,7 +501,7 @@ def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
             fin.append(f"h[{l}] = pca_fma({pf.ref(A2)}, s[{sidx[u2.id][0]}], {gn});")
 
     # trial residual: rprep once per trial point, residf per measurement
-    pr = _MeasPrinter(dag, dtype, fac, "R", dc)
+    pr = _MeasPrinter(dag, dtype, fac, "R", dc, exp_rec)
     pr.emit([mu])
     rf = list(pr.lines) + [f"return {resid(pr)};"]
     prr = _Printer(dag, dtype)
@@ -466,15 +510,19 @@ def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
 
     ind = lambda lines: ["    " + x for x in lines]
     nd = len(dc.entries) if dc is not None else 0
+    ne = max(len(pm.rec_terms), len(pr.rec_terms))
     out = ["  static constexpr bool FACTORED = true;",
            f"  static constexpr int NCT = {dag.nconst + nd};",
            f"  static constexpr int NS = {max(1, len(sidx))};",
-           f"  struct Pre {{ real a[{max(1, len(pm.pre_nodes))}]; real L[{n}]; }};",
+           f"  static constexpr int NE = {max(1, ne)};",
+           f"  static constexpr bool EXP_REC = {'true' if ne else 'false'};",
+           f"  struct Pre {{ real a[{max(1, len(pm.pre_nodes))}]; real L[{n}]; real fl; }};",
            f"  struct RPre {{ real a[{max(1, len(pr.pre_nodes))}]; }};",
            "  __device__ static __forceinline__ void prep(const real* __restrict__ p, real lam, "
            "Pre& P) {"] + ind(prep) + ["  }"]
     out += ["  __device__ static __forceinline__ void accf(const Pre& P, "
-            "const real* __restrict__ c, real y, real& f, real* s, real* q, real* dmp) {"]
+            "const real* __restrict__ c, real y, real& f, real* s, real* q, real* dmp, "
+            "int i, real* ex, real* qx) {"]
     out += ind(acc) + ["  }"]
     out += ["  __device__ static __forceinline__ void finish(const real* __restrict__ p, "
             "const real* L, const real* s, const real* q, const real* dmp, real* g, real* h, "
@@ -483,14 +531,14 @@ def _emit_factored(dag: ModelDAG, dtype: str, derive: bool = False):
     out += ["  __device__ static __forceinline__ void rprep(const real* __restrict__ p, RPre& R) {"]
     out += ind(rp) + ["  }"]
     out += ["  __device__ static __forceinline__ real residf(const RPre& R, "
-            "const real* __restrict__ c, real y) {"]
+            "const real* __restrict__ c, real y, int i, real* ex, real* qx) {"]
     out += ind(rf) + ["  }"]
     return out, (dc.entries if dc is not None else [])
 
 
 def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
                reassoc: bool = False, factored: bool = False,
-               derived: list | None = None) -> str:
+               derived: list | None = None, exp_rec: bool = False) -> str:
     """HIP source of the model struct (see module docstring).
 
     reassoc=False keeps the reference's association of the per-measurement
@@ -504,7 +552,9 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
     builds with voxel-uniform consts), the accumulation reads its const-only
     subexpressions from consts rows NC .. NCT-1, and the list receives their
     DerivedConsts entries (derived_values() computes the rows); NCT = NC
-    otherwise."""
+    otherwise. exp_rec=True (with ``derived``) evaluates exp(k*x) through the
+    exponential recurrence (pca_exp_rec, the kernel's PCA_EXP_REC), which
+    needs every such factor row k to be an arithmetic progression."""
     if factored and not reassoc:
         raise ValueError("the factored accumulation is a reassociated (fast) form")
     n, nh = dag.nparam, dag.nhes
@@ -576,7 +626,8 @@ def emit_model(dag: ModelDAG, struct_name: str, dtype: str = "float32",
     out += ["    " + s for s in body]
     out.append("  }")
     if factored:
-        lines, entries = _emit_factored(dag, dtype, derive=derived is not None)
+        lines, entries = _emit_factored(dag, dtype, derive=derived is not None,
+                                        exp_rec=exp_rec)
         out += lines
         if derived is not None:
             derived.extend(entries)

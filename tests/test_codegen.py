@@ -292,14 +292,23 @@ _FACTORED_SHIM = """
 typedef {real} real;
 static inline real pca_fma(real a, real b, real c) {{ return std::fma(a, b, c); }}
 static inline real pca_fmax(real a, real b) {{ return std::fmax(a, b); }}
-static inline real pca_floor_excess(real L, real uu) {{
-  return std::fmax(std::fma(-L, uu, (real)1e-4), (real)0);
+static inline real pca_vgpr(real x) {{ return x; }}
+static inline real pca_floor_excess(real L, real uu, real fl) {{
+  return std::fmax(std::fma(-L, uu, fl), (real)0);
 }}
 #define PCA_EXP(x) std::exp(x)
 #define PCA_LOG(x) std::log(x)
 #define PCA_POW(x, y) std::pow(x, y)
 #define PCA_EXPK(k, x) std::exp((k) * (x))
 #define PCA_EXPKD(ks, x) {expkd}
+// the prelude's exponential recurrence (csrc/prelude.hip), restated for the host
+#define PCA_EXP_REC {rec}
+template <int R>
+static inline real pca_exp_rec(int i, real& e, real& q, real k, real ks, real x) {{
+  if (i == 0) q = PCA_EXPKD(ks, x);
+  if (i % R == 0) e = PCA_EXPKD(k, x); else e = e * q;
+  return e;
+}}
 {plain}
 {derived}
 // one iteration's sums as the kernel forms them: prep, accf per measurement
@@ -311,10 +320,11 @@ static void drive(const real* p, const real* C, const real* y, int M, real lam, 
   typename Mdl::Pre P;
   Mdl::prep(p, lam, P);
   real s[Mdl::NS] = {{}}, q[NH] = {{}}, dmp[NP] = {{}}, f = 0;
+  real ex[Mdl::NE], qx[Mdl::NE];
   for (int i = 0; i < M; ++i) {{
     real c[Mdl::NCT + 1];
     for (int k = 0; k < Mdl::NCT; ++k) c[k] = C[k * M + i];
-    Mdl::accf(P, c, y[i], f, s, q, dmp);
+    Mdl::accf(P, c, y[i], f, s, q, dmp, i, ex, qx);
   }}
   real g[NP], h[NH], hld[NP];
   Mdl::finish(p, P.L, s, q, dmp, g, h, hld);
@@ -325,7 +335,7 @@ static void drive(const real* p, const real* C, const real* y, int M, real lam, 
   for (int i = 0; i < M; ++i) {{
     real c[Mdl::NCT + 1];
     for (int k = 0; k < Mdl::NCT; ++k) c[k] = C[k * M + i];
-    const real r = Mdl::residf(R, c, y[i]);
+    const real r = Mdl::residf(R, c, y[i], i, ex, qx);
     fr += r * r;
   }}
   int o = 0;
@@ -342,14 +352,18 @@ extern "C" void run_derived(const real* p, const real* C, const real* y, int M, 
 """
 
 
-def _build_factored_host(dag, dtype, exp2, tmp_path):
+def _build_factored_host(dag, dtype, exp2, tmp_path, exp_rec=0):
+    """The emitted factored members built for the host: run_plain (const-only
+    subexpressions per measurement) and run_derived (read from derived consts
+    rows; with ``exp_rec`` > 0 through the exponential recurrence)."""
     ent = []
     plain = hipgen.emit_model(dag, "Plain", dtype, reassoc=True, factored=True)
     derived = hipgen.emit_model(dag, "Derived", dtype, reassoc=True, factored=True,
-                                derived=ent)
+                                derived=ent, exp_rec=exp_rec > 0)
     real = "float" if dtype == "float32" else "double"
     expkd = "std::exp2((ks) * (x))" if exp2 else "std::exp((ks) * (x))"
-    src = _FACTORED_SHIM.format(real=real, expkd=expkd, plain=plain, derived=derived)
+    src = _FACTORED_SHIM.format(real=real, expkd=expkd, plain=plain, derived=derived,
+                                rec=max(1, exp_rec))
     cpp, so = tmp_path / "factored.cpp", tmp_path / "factored.so"
     cpp.write_text(src)
     subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-o", str(so), str(cpp)],
@@ -384,6 +398,102 @@ def test_factored_emitted_code_matches_fghhl(expr, pars, consts, tmp_path):
             fn(p.ctypes.data_as(dp), np.ascontiguousarray(C).ctypes.data_as(dp),
                y.ctypes.data_as(dp), M, lam, out.ctypes.data_as(dp))
             np.testing.assert_allclose(out, want, rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize("rec", [1, 3, 16])
+@pytest.mark.parametrize("expr,pars,consts", FACTOR_EXPRS)
+def test_exp_recurrence_matches_fghhl(expr, pars, consts, rec, tmp_path):
+    """The emitted factored members with the exponential recurrence (exp(k_i
+    x) directly every ``rec`` measurements, the previous value times exp(d x)
+    in between), built for the host in float64 on consts rows that are
+    arithmetic progressions: the reference's f, g, h, hld and trial SSR. A
+    model whose exponential factor is not an arithmetic progression in the
+    measurement (a product of two consts rows) is refused by
+    derived_values()."""
+    dag = hipgen.res_jac_hes(expr, pars, consts)
+    M, n, nc = 12, dag.nparam, dag.nconst
+    cs = np.array([np.linspace(0.2 + 0.1 * k, 2.0 - 0.2 * k, M) for k in range(nc)])
+    cs = cs.reshape(nc, M)
+    L, ent = _build_factored_host(dag, "float64", False, tmp_path, exp_rec=rec)
+    kinds = [k for k, _ in ent]
+    if "expk" in kinds:
+        assert kinds.count("expstep") == kinds.count("expk")
+    try:
+        table = np.concatenate([cs, hipgen.derived_values(ent, dag, cs, False)]).ravel()
+    except ValueError as err:
+        assert "arithmetic progression" in str(err)
+        rows = hipgen.derived_values([e for e in ent if e[0] == "expk"], dag, cs, False)
+        dev = [np.max(np.abs(r - np.linspace(r[0], r[-1], M))) for r in rows]
+        assert max(dev) > hipgen.EXP_REC_TOL * np.max(np.abs(rows))
+        return
+    if nc == 0:
+        table = np.zeros(1)
+    rng = np.random.default_rng(17)
+    dp = ctypes.POINTER(ctypes.c_double)
+    for lam in (1e-6, 0.37, 1e3):
+        p = np.array([rng.uniform(0.3, 1.5) for _ in dag.pars])
+        y = rng.uniform(0.0, 2.0, M)
+        f, g, h, hld = _fghhl_ref(dag, p, list(cs), y, lam)
+        want = np.concatenate([[f, f], g, h, hld])
+        out = np.zeros(2 + 2 * n + dag.nhes)
+        L.run_derived.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_double, dp]
+        L.run_derived(p.ctypes.data_as(dp), np.ascontiguousarray(table).ctypes.data_as(dp),
+                      y.ctypes.data_as(dp), M, lam, out.ctypes.data_as(dp))
+        np.testing.assert_allclose(out, want, rtol=1e-10, atol=1e-11)
+
+
+def test_arithmetic_step():
+    """The common difference of an exponential-recurrence row: exact rows and
+    float32-rounded linspace rows pass, a row with one value moved by more
+    than the rounding of float32 consts does not; a single measurement has
+    step 0."""
+    v = np.linspace(0.0, 800.0, 16)
+    assert hipgen.arithmetic_step(v) == pytest.approx(800.0 / 15, rel=1e-15)
+    v32 = np.linspace(0.0, 800.0, 16).astype(np.float32).astype(np.float64)
+    assert hipgen.arithmetic_step(v32) == pytest.approx(800.0 / 15, rel=1e-7)
+    assert hipgen.arithmetic_step(np.array([3.0])) == 0.0
+    bad = v.copy()
+    bad[7] += 800.0 * 2.0 ** -20
+    with pytest.raises(ValueError, match="arithmetic progression"):
+        hipgen.arithmetic_step(bad)
+    with pytest.raises(ValueError, match="arithmetic progression"):
+        hipgen.arithmetic_step(np.array([0.0, 10.0, 20.0, 50.0, 100.0]))
+
+
+@pytest.mark.parametrize("name", ["ivim", "t2"])
+def test_exp_recurrence_float32(name, tmp_path):
+    """Float32 exp2 form with the exponential recurrence (anchors every 8
+    measurements) on the benchmark protocols (16 b-values 0-800 s/mm^2, 32
+    echoes 10-320 ms): the float64 reference at the float32 bar, and within a
+    few ulp-level errors of the direct float32 evaluation."""
+    m = MODELS[name]
+    dag = hipgen.res_jac_hes(m.expr, m.pars, m.consts)
+    (tmp_path / "r").mkdir()
+    (tmp_path / "d").mkdir()
+    Lr, ent_r = _build_factored_host(dag, "float32", True, tmp_path / "r", exp_rec=8)
+    Ld, ent_d = _build_factored_host(dag, "float32", True, tmp_path / "d")
+    M = 16 if name == "ivim" else 32
+    cs = (np.linspace(0.0, 800.0, M) * 1e-3 if name == "ivim"
+          else np.linspace(10.0, 320.0, M))[None, :]
+    p = np.array({"ivim": [1.0, 0.2, 3.0, 0.8], "t2": [1.0, 60.0]}[name])
+    rng = np.random.default_rng(4)
+    f0, _, _, _ = _fghhl_ref(dag, p, list(cs), np.zeros(M), 0.1)
+    y = np.sqrt(f0 / M) * rng.uniform(0.5, 1.5, M)
+    f, g, h, hld = _fghhl_ref(dag, p, list(cs), y, 0.1)
+    want = np.concatenate([[f, f], g, h, hld])
+    fp = ctypes.POINTER(ctypes.c_float)
+    outs = []
+    for L, ent in ((Lr, ent_r), (Ld, ent_d)):
+        table = np.concatenate([cs, hipgen.derived_values(ent, dag, cs, True)]).astype(np.float32)
+        out = np.zeros(len(want), dtype=np.float32)
+        L.run_derived.argtypes = [fp, fp, fp, ctypes.c_int, ctypes.c_float, fp]
+        p32, y32 = p.astype(np.float32), y.astype(np.float32)
+        L.run_derived(p32.ctypes.data_as(fp),
+                      np.ascontiguousarray(table.ravel()).ctypes.data_as(fp),
+                      y32.ctypes.data_as(fp), M, 0.1, out.ctypes.data_as(fp))
+        np.testing.assert_allclose(out, want, rtol=2e-4, atol=2e-4 * np.max(np.abs(want)))
+        outs.append(out.astype(np.float64))
+    np.testing.assert_allclose(outs[0], outs[1], rtol=2e-5, atol=2e-5 * np.max(np.abs(want)))
 
 
 def test_derived_consts_of_builtin_models():

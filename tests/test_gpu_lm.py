@@ -83,7 +83,14 @@ FAST_BUILDS = {"fast": dict(fast_exp=True, fast_div=True),
                "fast-reassoc": dict(fast_exp=True, fast_div=True, reassoc=True),
                "fast-factored": dict(fast_exp=True, fast_div=True, reassoc=True, factored=True),
                "fast-factored-fenced": dict(fast_exp=True, fast_div=True, reassoc=True,
-                                            factored=True, sched_group=4)}
+                                            factored=True, sched_group=4),
+               # exponential recurrence: every protocol here (16 b-values
+               # 0-800, 32 echoes 10-320 ms, 16 b-values 50-800) is an
+               # arithmetic progression
+               "fast-factored-rec8": dict(fast_exp=True, fast_div=True, reassoc=True,
+                                          factored=True, exp_rec=8),
+               "fast-factored-fenced-rec4": dict(fast_exp=True, fast_div=True, reassoc=True,
+                                                 factored=True, sched_group=4, exp_rec=4)}
 
 
 @pytest.mark.parametrize("build", sorted(FAST_BUILDS))
