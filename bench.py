@@ -593,8 +593,12 @@ def run_dki(args, stream):
              "lm_kernel_ms": k_ms, "lm_tflops": tflops,
              "lm_frac_of_fp32_peak": tflops * 1e12 / PEAK_FP32,
              "excluded_voxels": int((~ok).sum())}
-    kernel = "pca_lm_glm_mfma (wide GLM kernel, Hessian on the matrix cores)" \
-        if kn.get("mfma") else "pca_lm_glm (wide GLM kernel)"
+    if kn.get("quad"):
+        kernel = "pca_lm_glm_quad (wide GLM kernel, four lanes per voxel)"
+    elif kn.get("mfma"):
+        kernel = "pca_lm_glm_mfma (wide GLM kernel, Hessian on the matrix cores)"
+    else:
+        kernel = "pca_lm_glm (wide GLM kernel)"
     if args.dki_plain_only:
         return {"workload": "DKI 22 parameters, 2,097,152 voxels, 61 measurements, SNR 30",
                 "voxels": N, "kernel": kernel, "tuning": kn, "plain_lm": plain}

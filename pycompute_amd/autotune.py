@@ -28,6 +28,9 @@ of the caller's own problem:
   * tile:          wide-model kernel register/LDS split (wide_candidates())
   * mfma:          generalised linear models with n > 8: the Hessian on the
                    matrix cores (pca_lm_glm_mfma) instead of VALU tiles
+  * quad:          generalised linear models with 9 to 24 parameters: four
+                   lanes per voxel (pca_lm_glm_quad); quad_waves its occupancy
+                   hint
 
 The fastest candidate whose results agree with the reference-rounding build on
 the sample (final SSR within 1e-4 relative on at least 98% of voxels, and no
@@ -78,8 +81,9 @@ def candidates(nparam: int, allow_fast: bool):
     """The single-model knob sets. The FMA-folded build runs with an occupancy
     hint of 2 waves per SIMD too: it needs a few registers more than 256,
     which would otherwise halve occupancy. The factored build runs unfenced
-    and with a fence every 4 measurements (about 120 registers, 4 waves per
-    SIMD)."""
+    and with a fence every 4 measurements (about 105 registers, 4 waves per
+    SIMD), and with the exponential recurrence also fenced every 1 or 2
+    measurements at 5 waves per SIMD."""
     out = []
     fast = [(False, False, False, 0, False, 0)]
     if allow_fast:
@@ -106,6 +110,17 @@ def candidates(nparam: int, allow_fast: bool):
                      "exp_rec": 0, "uc_lds": False}
                 if c not in out:
                     out.append(c)
+    if allow_fast and nparam <= 8:
+        # the recurrence builds fenced every one or two measurements with an
+        # occupancy hint of 5 waves per SIMD (about 94 registers, no spills
+        # for the 4-parameter models)
+        for sg in (1, 2):
+            for rec in (4, 8):
+                for chunk, mc in ((8, 0), (16, 0), (24, 0), (16, 4)):
+                    out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                                "fast_exp": True, "fast_div": True, "reassoc": True,
+                                "waves_per_eu": 5, "factored": True, "sched_group": sg,
+                                "vpl": 1, "exp_rec": rec, "uc_lds": True})
     return out
 
 
@@ -150,8 +165,14 @@ def wide_candidates(allow_fast: bool = True):
     for fe, fd, ra in fast:
         for tile in (100, 136, 170):
             out.append({"tile": tile, "fast_exp": fe, "fast_div": fd, "reassoc": ra,
-                        "mfma": False})
-        out.append({"tile": 136, "fast_exp": fe, "fast_div": fd, "reassoc": ra, "mfma": True})
+                        "mfma": False, "quad": 0, "quad_waves": 0})
+        out.append({"tile": 136, "fast_exp": fe, "fast_div": fd, "reassoc": ra, "mfma": True,
+                    "quad": 0, "quad_waves": 0})
+        # four lanes per voxel (generalised linear models with 9 to 24
+        # parameters), with and without an occupancy hint of 3 waves per SIMD
+        for qw in (0, 3):
+            out.append({"tile": 136, "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                        "mfma": False, "quad": 4, "quad_waves": qw})
     return out
 
 

@@ -145,7 +145,7 @@ class SecondOrderLevenbergMarquardt(HipFunction):
                  waves_per_eu: int = 0, fast_div: bool = False, wide: bool | None = None,
                  tile: int = 136, reassoc: bool = False, sched_group: int = 0,
                  factored: bool = False, vpl: int = 1, mfma: bool = False, exp_rec: int = 0,
-                 uc_lds: bool = False, **kwargs):
+                 uc_lds: bool = False, quad: int = 0, quad_waves: int = 0, **kwargs):
         if (args and isinstance(args[0], str)) or "expr" in kwargs:
             # notebook form: (expr, pars_str, consts_str, ndata, dtype, write_to_file)
             a = _bind(args, kwargs, ("expr", "pars_str", "consts_str", "ndata", "dtype",
@@ -204,8 +204,12 @@ class SecondOrderLevenbergMarquardt(HipFunction):
         # registers hold (n > 8): csrc/lm_glm.hip for generalised linear
         # models with voxel-uniform consts, csrc/lm_wide.hip (Hessian kept in
         # LDS) for any other model. None = use them whenever they apply.
-        # mfma: the GLM kernel with the Hessian on the matrix cores (float32)
+        # mfma: the GLM kernel with the Hessian on the matrix cores (float32);
+        # quad: the GLM kernel with 4 lanes per voxel (quad=4 or True; float32,
+        # n <= 24), quad_waves its occupancy hint (wavefronts per SIMD)
         self.wide, self.tile, self.mfma = wide, int(tile), bool(mfma)
+        self.quad = 4 if quad is True else int(quad)
+        self.quad_waves = int(quad_waves)
         self._glm, self._glm_tab, self._wide_prog = None, None, None
         # more consts than the kernel-argument block holds: per-voxel layout
         self._uc_fits = self.nconst * self.ndata <= _lib.PCA_UC_MAX
@@ -214,10 +218,14 @@ class SecondOrderLevenbergMarquardt(HipFunction):
             self.funcid = self._spec_uniform.funcid()
             if wide is not False:
                 from . import wide as _wide
-                self._glm = _wide.glm_program(self._spec_uniform, self.tile, self.mfma)
+                self._glm = _wide.glm_program(self._spec_uniform, self.tile, self.mfma,
+                                              self.quad, self.quad_waves)
                 if self.mfma and self._glm is None:
                     raise ValueError("mfma needs a float32 generalised linear model with n > 8 "
                                      "whose tables fit in the LDS")
+                if self.quad and self._glm is None:
+                    raise ValueError("quad needs a float32 generalised linear model with "
+                                     "9 to 24 parameters")
                 if wide and self._glm is None and _wide.wide_program(
                         self._spec(not self._uc_fits), self.tile) is None:
                     raise ValueError("the wide kernels need n > 8 and a packed Hessian that "

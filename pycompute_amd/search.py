@@ -166,7 +166,8 @@ class SecondOrderRandomSearch(_SearchBase):
         lm_kw = {k: kwargs.pop(k) for k in list(kwargs)
                  if k in ("compaction", "chunk", "max_chunks", "fast_exp", "fast_div", "reassoc",
                           "unroll", "waves_per_eu", "sched_group", "factored", "vpl", "wide",
-                          "exp_rec", "uc_lds")}
+                          "exp_rec", "uc_lds", "tile", "mfma", "quad",
+                          "quad_waves")}
         super().__init__(*args, seed=seed, block=block, strict=strict, **kwargs)
         if batch is not None and int(batch) < 1:
             raise ValueError("batch must be at least 1")

@@ -56,14 +56,26 @@ def mfma_lds_bytes(n: int, M: int) -> int:
     return M * 4 * rs * 4 + 4 * M * 64 * 4 + 4 * 64 * (n + 1) * 4
 
 
-class GLMProgram:
-    """pca_lm_glm (Hessian tiles accumulated on the VALU, parked in LDS) or,
-    with mfma=True, pca_lm_glm_mfma (Hessian accumulated on the matrix
-    cores, one voxel per lane; float only)."""
+QUAD_MAX_PARAMS = 24   # pca_lm_glm_quad: rows of the pivoted system per voxel
 
-    def __init__(self, spec: lm.LMSpec, tile_max: int = DEFAULT_TILE, mfma: bool = False):
+
+class GLMProgram:
+    """pca_lm_glm (Hessian tiles accumulated on the VALU, parked in LDS), or
+    with mfma=True pca_lm_glm_mfma (Hessian accumulated on the matrix cores,
+    one voxel per lane; float only), or with quad = 4 (True)
+    pca_lm_glm_quad (four lanes per voxel, the rows of the pivoted system
+    spread over them; float only, at most 24 parameters; quad_waves is its
+    occupancy hint)."""
+
+    def __init__(self, spec: lm.LMSpec, tile_max: int = DEFAULT_TILE, mfma: bool = False,
+                 quad: int = 0, quad_waves: int = 0):
         self.spec = spec
         self.mfma = bool(mfma)
+        self.quad = quad_lanes(quad)
+        # occupancy hint of the quad kernel (wavefronts per SIMD; 0 = none)
+        self.quad_waves = int(quad_waves) if self.quad else 0
+        if self.mfma and self.quad:
+            raise ValueError("mfma and quad are different GLM kernels; pick one")
         self.dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
         self.form = glm.glm_form(self.dag)
         if self.form is None:
@@ -81,8 +93,15 @@ class GLMProgram:
             if mfma_lds_bytes(n, spec.M) > LDS_LIMIT:
                 raise ValueError(f"the matrix-core GLM kernel needs {mfma_lds_bytes(n, spec.M)} "
                                  f"bytes of LDS for M = {spec.M} (limit {LDS_LIMIT})")
+        if self.quad:
+            if spec.dtype != "float32":
+                raise ValueError("the quad GLM kernel is float32 only")
+            if n > QUAD_MAX_PARAMS:
+                raise ValueError(f"the quad GLM kernel takes at most {QUAD_MAX_PARAMS} "
+                                 f"parameters, got {n}")
         self.stride = glm.table_stride(n)
-        self.funcid = spec.funcid() + f"_glm{self.tile_max}" + ("_mfma" if self.mfma else "")
+        self.funcid = spec.funcid() + f"_glm{self.tile_max}" + ("_mfma" if self.mfma else "") \
+            + (f"_quad{self.quad}w{self.quad_waves}" if self.quad else "")
         head = [f"// generated: {self.funcid}", f"// model: {spec.expr}",
                 f"#define PCA_NP {n}", f"#define PCA_M {spec.M}", f"#define PCA_TS {self.stride}",
                 f"#define PCA_TILE_MAX {self.tile_max}", f"#define PCA_BLOCK {spec.block}"]
@@ -94,10 +113,16 @@ class GLMProgram:
             head.append("#define PCA_GLM_IDENTITY 1")
         if spec.reassoc:
             head.append("#define PCA_FAST_SOLVE 1")
+        if self.quad:
+            head.append(f"#define PCA_LPV {self.quad}")
+            if self.quad_waves:
+                head.append(f"#define PCA_QUAD_WAVES {self.quad_waves}")
+            kernels = [read_csrc("lm_glm_quad.hip")]
+        else:
+            kernels = [read_csrc("lm_glm.hip")] + ([read_csrc("lm_glm_mfma.hip")]
+                                                   if self.mfma else [])
         self.source = "\n".join(["\n".join(head), read_csrc("strict_math.h"),
-                                 read_csrc("prelude.hip"), read_csrc("lm_solver.hip"),
-                                 read_csrc("lm_glm.hip")]
-                                + ([read_csrc("lm_glm_mfma.hip")] if self.mfma else []))
+                                 read_csrc("prelude.hip"), read_csrc("lm_solver.hip")] + kernels)
         options = list(BASE_OPTIONS)
         if spec.fast_div:
             options = [o for o in options if o != "-fhip-fp32-correctly-rounded-divide-sqrt"]
@@ -111,6 +136,9 @@ class GLMProgram:
         mfma, the pair (table, entry-product table)."""
         c = np.asarray(uc, np.float64).reshape(self.spec.nconst, self.spec.M)
         tab = glm.coefficient_table(self.form, self.dag, c)
+        if self.quad:
+            return torch.tensor(quad_table(tab, self.spec.nparam, self.quad),
+                                dtype=torch.float32, device=device)
         t = torch.tensor(tab, dtype=lm.torch_dtype(self.spec.dtype), device=device)
         if not self.mfma:
             return t
@@ -119,6 +147,20 @@ class GLMProgram:
 
     def launch(self, args: _lib.LMArgs, tab, stream=None):
         s = lm.stream_handle() if stream is None else stream
+        if self.quad:
+            L = self.quad
+            if (tab.shape != (self.spec.M, L, quad_slots(self.spec.nparam, L))
+                    or tab.dtype != torch.float32 or not tab.is_contiguous()):
+                raise ValueError("the quad kernel needs the float32 table of "
+                                 "GLMProgram.table()")
+            t = ctypes.c_void_p(tab.data_ptr())
+            params = (ctypes.c_void_p * 2)(ctypes.cast(ctypes.pointer(args), ctypes.c_void_p),
+                                           ctypes.cast(ctypes.pointer(t), ctypes.c_void_p))
+            vpb = 256 // L                        # voxels of L lanes per workgroup
+            grid = (int(args.N) + vpb - 1) // vpb
+            _lib.check(_lib.lib().pca_launch(self.module.kernel("pca_lm_glm_quad"), grid, 1, 1,
+                                             256, 1, 1, 0, s, params), "pca_lm_glm_quad")
+            return
         if self.mfma:
             t, t2 = (ctypes.c_void_p(x.data_ptr()) for x in tab)
             params = (ctypes.c_void_p * 3)(ctypes.cast(ctypes.pointer(args), ctypes.c_void_p),
@@ -134,6 +176,50 @@ class GLMProgram:
         grid = (int(args.N) + 63) // 64
         _lib.check(_lib.lib().pca_launch(self.module.kernel("pca_lm_glm"), grid, 1, 1, 64, 1, 1,
                                          0, s, params), "pca_lm_glm")
+
+
+def quad_lanes(quad) -> int:
+    """Lanes per voxel of the quad GLM kernel: 0 (off) or 4 (True). The
+    kernel also compiles for 2 lanes per voxel (PCA_LPV 2), but with 11 row
+    groups per lane the compiler keeps the solve's loops rolled over arrays in
+    scratch, and the 22-parameter fit ran 12x slower; the host does not offer
+    it."""
+    lanes = 4 if quad is True else int(quad)
+    if lanes not in (0, 4):
+        raise ValueError(f"quad must be 0 or 4 lanes per voxel, got {quad}")
+    return lanes
+
+
+def quad_perm(n: int, a: int) -> int:
+    """Parameter of row a of the pivoted system: the reference's diagonal
+    pivot is the fixed rotation [n-1, 0, 1, ..., n-2] (lm_solver.hip perm())."""
+    return n - 1 if a == 0 else a - 1
+
+
+def quad_slots(n: int, L: int) -> int:
+    """Floats per (measurement, lane) of pca_lm_glm_quad's table: the lane's
+    NT = ceil(n / L) rows, a0, padded to a multiple of 4."""
+    return (-(-n // L) + 1 + 3) // 4 * 4
+
+
+def quad_table(tab: np.ndarray, n: int, L: int = 4) -> np.ndarray:
+    """(M, L, quad_slots) float64 table of pca_lm_glm_quad with L lanes per
+    voxel from the coefficient table: [i, r, t] = A_{perm(L t + r)} of
+    measurement i for L t + r < n (else 0), [i, r, NT] = a0 (NT = ceil(n /
+    L)), zero padding."""
+    if n > QUAD_MAX_PARAMS:
+        raise ValueError(f"at most {QUAD_MAX_PARAMS} parameters")
+    if L not in (2, 4):
+        raise ValueError("2 or 4 lanes per voxel")
+    M, nt = tab.shape[0], -(-n // L)
+    out = np.zeros((M, L, quad_slots(n, L)))
+    for r in range(L):
+        for t in range(nt):
+            a = L * t + r
+            if a < n:
+                out[:, r, t] = tab[:, quad_perm(n, a)]
+        out[:, r, nt] = tab[:, 2 * n]
+    return out
 
 
 def entry_products(tab: np.ndarray, n: int) -> np.ndarray:
@@ -252,18 +338,24 @@ def wide_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE) -> WideProgram |
 
 
 def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE,
-                mfma: bool = False) -> GLMProgram | None:
+                mfma: bool = False, quad: int = 0, quad_waves: int = 0) -> GLMProgram | None:
     """The wide-kernel program for ``spec``, or None when the model is not a
     GLM, is small enough for the register kernel, or parks more LDS than a
     workgroup may hold at this tile (with mfma: is not float32, or its
-    tables exceed the LDS)."""
+    tables exceed the LDS; with quad (4 lanes per voxel): is not float32 or
+    has more than 24 parameters)."""
     if (spec.nparam < WIDE_MIN_PARAMS or spec.strict
             or lds_bytes(spec.nparam, tile_max, spec.dtype) > LDS_LIMIT):
         return None
     if mfma and (spec.dtype != "float32"
                  or mfma_lds_bytes(spec.nparam, spec.M) > LDS_LIMIT):
         return None
-    key = (spec, tile_max, bool(mfma))
+    quad = quad_lanes(quad)
+    if quad and (mfma or spec.dtype != "float32" or spec.nparam > QUAD_MAX_PARAMS
+                 or spec.M * quad * quad_slots(spec.nparam, quad) * 4 > LDS_LIMIT):
+        return None
+    quad_waves = int(quad_waves) if quad else 0
+    key = (spec, tile_max, bool(mfma), quad, quad_waves)
     with _lock:
         if key in _programs:
             return _programs[key]
@@ -273,7 +365,7 @@ def glm_program(spec: lm.LMSpec, tile_max: int = DEFAULT_TILE,
             if key in _programs:
                 return _programs[key]
         dag = hipgen.res_jac_hes(spec.expr, spec.pars, spec.consts)
-        prog = GLMProgram(spec, tile_max, mfma) if glm.glm_form(dag) else None
+        prog = GLMProgram(spec, tile_max, mfma, quad, quad_waves) if glm.glm_form(dag) else None
         with _lock:
             _programs[key] = prog
         return prog

@@ -3,7 +3,7 @@ coefficient table (no GPU)."""
 import numpy as np
 import pytest
 
-from pycompute_amd import synth
+from pycompute_amd import synth, wide
 from pycompute_amd.models import DKI, IVIM, SE, T2, dki_design
 from pycompute_amd.sym import glm, hipgen
 
@@ -110,3 +110,32 @@ def test_mfma_entry_product_table_layout():
     # LDS of a 256-thread workgroup: the table, the samples and g over the solve
     assert wide.mfma_lds_bytes(22, 61) == 61 * 4 * 68 * 4 + 4 * 61 * 64 * 4 + 4 * 64 * 23 * 4
     assert wide.mfma_lds_bytes(22, 61) <= wide.LDS_LIMIT < wide.mfma_lds_bytes(22, 80)
+
+
+def test_quad_table_layout():
+    """pca_lm_glm_quad's table for L lanes per voxel: [i, r, t] is column
+    perm(L t + r) of the coefficient table (the reference's fixed pivot
+    rotation), 0 past the last parameter, a0 in slot ceil(n / L) of every
+    lane, zero padding to a multiple of 4 floats."""
+    rng = np.random.default_rng(3)
+    for L in (2, 4):
+        for n in (9, 11, 22, 24):
+            tab = rng.standard_normal((5, glm.table_stride(n)))
+            q = wide.quad_table(tab, n, L)
+            nt = -(-n // L)
+            assert q.shape == (5, L, wide.quad_slots(n, L)) and q.shape[2] % 4 == 0
+            assert q.shape[2] >= nt + 1
+            for r in range(L):
+                for t in range(q.shape[2]):
+                    a = L * t + r
+                    if t < nt:
+                        want = tab[:, n - 1 if a == 0 else a - 1] if a < n else 0.0
+                    else:
+                        want = tab[:, 2 * n] if t == nt else 0.0
+                    np.testing.assert_array_equal(q[:, r, t], want)
+    with pytest.raises(ValueError):
+        wide.quad_table(np.zeros((5, glm.table_stride(25))), 25)
+    for bad in (2, 3):
+        with pytest.raises(ValueError):
+            wide.quad_lanes(bad)
+    assert wide.quad_lanes(True) == 4 and wide.quad_lanes(False) == 0
