@@ -95,7 +95,7 @@ def candidates(nparam: int, allow_fast: bool):
         # every 4 or 8 measurements) and the consts tables in LDS
         recs = ((0, False), (4, True), (8, True)) if fac and nparam <= 8 else ((0, False),)
         for rec, lds in recs:
-            for chunk, mc in ((8, 0), (16, 0), (24, 0), (16, 4)):
+            for chunk, mc in ((16, 4), (24, 0), (16, 0), (8, 0)):
                 for block in (256, 128):
                     out.append({"block": block, "chunk": chunk, "max_chunks": mc, "unroll": 0,
                                 "fast_exp": fe, "fast_div": fd, "reassoc": ra,
@@ -116,7 +116,7 @@ def candidates(nparam: int, allow_fast: bool):
         # for the 4-parameter models)
         for sg in (1, 2):
             for rec in (4, 8):
-                for chunk, mc in ((8, 0), (16, 0), (24, 0), (16, 4)):
+                for chunk, mc in ((16, 4), (24, 0), (16, 0), (8, 0)):
                     out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
                                 "fast_exp": True, "fast_div": True, "reassoc": True,
                                 "waves_per_eu": 5, "factored": True, "sched_group": sg,
@@ -152,6 +152,11 @@ def batch_candidates(allow_fast: bool = True):
                                 "fast_exp": fe, "fast_div": fd, "reassoc": ra,
                                 "waves_per_eu": 0, "factored": fac, "sched_group": 4,
                                 "vpl": 1, "exp_rec": rec, "uc_lds": lds})
+                # the recurrence fenced every 2 measurements at 5 waves per SIMD
+                out.append({"block": 256, "chunk": chunk, "max_chunks": mc, "unroll": 0,
+                            "fast_exp": fe, "fast_div": fd, "reassoc": ra,
+                            "waves_per_eu": 5, "factored": fac, "sched_group": 2,
+                            "vpl": 1, "exp_rec": 4, "uc_lds": True})
     return out
 
 
@@ -192,7 +197,9 @@ def pick(results: list) -> dict:
     """The winning entry of timed candidates ({"knobs", "ms", "agree",
     "worse", "better"}, in candidate order): the fastest acceptable one, where
     timings within 1% of the fastest are a tie (run-to-run noise is of that
-    order), won by the earliest candidate so that repeated tunings agree."""
+    order), won by the earliest candidate so that repeated tunings agree.
+    candidates() lists the chunk schedules with fewer chunks first, so a tie
+    goes to the one that moves fewer queue bytes through HBM."""
     ok = [r for r in results if acceptable(r)]
     fastest = min(r["ms"] for r in ok)
     return next(r for r in ok if r["ms"] <= fastest * 1.01)
