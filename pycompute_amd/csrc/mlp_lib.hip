@@ -234,15 +234,31 @@ __global__ void k_mse_grad(const float* Y, const float* T, float* dY, float* los
   if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, acc);
 }
 
-// out[n] += sum_m X[m * ld + n] (bias gradients), rows split over blocks
-__global__ void k_colsum(const float* X, long long ld, int M, int N, float* out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// out[n] += sum_m X[m * ld + n] (bias gradients). Block (x, y) sums rows
+// chunk y of the column tile [256 x, 256 x + nc): its 256 threads form
+// 256 / nc row groups of nc adjacent columns (each group reads contiguous row
+// segments), the groups' partial sums meet in LDS, and one atomic per column
+// and block adds them to out. Every thread works whatever N is (4-wide
+// output layers included).
+__global__ void __launch_bounds__(256) k_colsum(const float* X, long long ld, int M, int N,
+                                               float* out) {
+  __shared__ float red[256];
+  const int c0 = blockIdx.x * 256;
+  const int nc = min(256, N - c0);
+  const int groups = 256 / nc;
+  const int t = threadIdx.x, g = t / nc, c = t % nc;
   const int rows = (M + gridDim.y - 1) / gridDim.y;
   const int mb = blockIdx.y * rows, me = min(M, mb + rows);
   float acc = 0.f;
-  for (int m = mb; m < me; ++m) acc += X[(long long)m * ld + n];
-  atomicAdd(&out[n], acc);
+  if (g < groups)
+    for (int m = mb + g; m < me; m += groups) acc += X[(long long)m * ld + c0 + c];
+  red[t] = acc;   // threads past groups * nc hold 0 and are not read
+  __syncthreads();
+  if (t < nc) {
+    float sum = 0.f;
+    for (int q = 0; q < groups; ++q) sum += red[q * nc + t];
+    atomicAdd(&out[c0 + t], sum);
+  }
 }
 
 // Adam (Kingma and Ba 2015) with bias correction, on a flat parameter vector.
@@ -599,7 +615,8 @@ extern "C" int pca_mlp_mse_grad(const float* Y, const float* T, float* dY, float
 extern "C" int pca_mlp_colsum(const float* X, int64_t ld, int32_t M, int32_t N, float* out,
                               void* stream) {
   if (!X || !out || M <= 0 || N <= 0) return pca_fail(PCA_ERR_ARG, "pca_mlp_colsum: bad argument");
-  dim3 grid((N + 255) / 256, std::min(256, (M + 127) / 128));
+  // chunks of 64 rows (at most 1024 chunks): a few dependent loads per thread
+  dim3 grid((N + 255) / 256, std::min(1024, (M + 63) / 64));
   hipLaunchKernelGGL(k_colsum, grid, dim3(256), 0, (hipStream_t)stream, X, (long long)ld, M, N,
                      out);
   return check_launch();

@@ -56,6 +56,25 @@ def test_gemm_epilogues(gpu):
                                rtol=1e-4, atol=1e-4)
 
 
+@pytest.mark.parametrize("M,N,ld", [(1, 4, 4), (63, 22, 22), (8192, 4, 4), (8192, 64, 64),
+                                    (777, 300, 301), (100, 256, 260)])
+def test_colsum_matches_float64(gpu, M, N, ld):
+    """The bias-gradient column sums (out += X.sum(0) over the first N of ld
+    columns) against float64, with several column tiles, strided rows and
+    more row chunks than rows."""
+    from pycompute_amd import _lib
+    g = torch.Generator(device="cuda").manual_seed(M + N)
+    X = torch.randn((M, ld), device="cuda", generator=g)
+    out = torch.randn(N, device="cuda", generator=g)
+    want = out.double() + X[:, :N].double().sum(0)
+    L = mlp._bind()
+    _lib.check(L.pca_mlp_colsum(X.data_ptr(), ld, M, N, out.data_ptr(),
+                                torch.cuda.current_stream().cuda_stream), "colsum")
+    torch.cuda.synchronize()
+    err = (out.double() - want).abs().max().item()
+    assert err <= 1e-5 * max(1.0, float(np.sqrt(M))), err
+
+
 def _small_net(seed=0):
     net = mlp.ivim_mlp(synth.B_C2, seed=seed)
     return net
