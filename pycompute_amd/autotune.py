@@ -217,25 +217,31 @@ def tune(make_solver, setup, model_expr: str, M: int, dtype: str, allow_fast: bo
     arch = torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName
     k = key(model_expr, M, dtype, allow_fast, arch)
     db = _load()
+
+    def applicable(cand):
+        """The solver of ``cand`` bound to the sample, or the ValueError of a
+        knob set that does not apply to this problem (the exponential
+        recurrence on a protocol that is not an arithmetic progression)."""
+        try:
+            s = make_solver(compaction=True, **cand)
+            setup(s)
+            return s
+        except ValueError as err:
+            return err
+
     if not force and k in db:
         best = dict(db[k]["best"])
-        try:
-            # a knob set that does not apply to this problem's protocol (the
-            # exponential recurrence) is tuned again
-            make_solver(compaction=True, **best)
+        if not isinstance(applicable(best), ValueError):
             return best
-        except ValueError:
-            pass
     results = []
     ref_f = None
     for cand in (cands or candidates(0, allow_fast)):
-        try:
-            s = make_solver(compaction=True, **cand)
-        except ValueError as err:
+        s = applicable(cand)
+        if isinstance(s, ValueError):
             if ref_f is None:
-                raise
+                raise s
             if log:
-                log(f"autotune {cand} skipped: {err}")
+                log(f"autotune {cand} skipped: {s}")
             continue
         ts = []
         for r in range(reps + 1):

@@ -203,3 +203,63 @@ def test_chunk_schedule():
                 s = chunk_schedule(it, k, m)
                 assert sum(s) == it and all(x >= 1 for x in s)
                 assert m == 0 or len(s) <= m
+
+
+def test_autotune_skips_candidates_that_do_not_apply(monkeypatch):
+    """tune() skips a knob set whose solver cannot be bound to the sample
+    (ValueError at construction or setup, as the exponential recurrence on a
+    protocol that is not an arithmetic progression raises), re-tunes when the
+    cached choice no longer applies, and re-raises for the reference
+    candidate. CUDA timing and the cache file are replaced by stand-ins."""
+    import types
+    import torch
+    from pycompute_amd import autotune
+
+    class Ev:
+        t = 0.0
+
+        def __init__(self, enable_timing=True):
+            pass
+
+        def record(self):
+            Ev.t += 1.0
+            self.at = Ev.t
+
+        def elapsed_time(self, other):
+            return other.at - self.at
+
+    monkeypatch.setattr(torch.cuda, "get_device_properties",
+                        lambda d: types.SimpleNamespace(gcnArchName="gfx950"))
+    monkeypatch.setattr(torch.cuda, "current_device", lambda: 0)
+    monkeypatch.setattr(torch.cuda, "Event", Ev)
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda: None)
+    db = {}
+    monkeypatch.setattr(autotune, "_load", lambda: db)
+    monkeypatch.setattr(autotune, "_store", lambda d: None)
+
+    class Solver:
+        def __init__(self, **kn):
+            self.kn = kn
+
+        def run(self, iters):
+            Ev.t += 0.5 if self.kn.get("exp_rec") else 2.0   # the recurrence "is faster"
+
+        def final_f(self):
+            return torch.ones((1, 8))
+
+    def make(**kn):
+        return Solver(**kn)
+
+    def setup(s):
+        if s.kn.get("exp_rec") and not uniform[0]:
+            raise ValueError("not an arithmetic progression")
+
+    cands = [{"exp_rec": 0, "k": 0}, {"exp_rec": 4, "k": 1}, {"exp_rec": 0, "k": 2}]
+    uniform = [True]
+    best = autotune.tune(make, setup, "x", 4, "float32", cands=cands, reps=1)
+    assert best["exp_rec"] == 4
+    uniform[0] = False      # same key, a protocol the cached choice does not fit
+    best = autotune.tune(make, setup, "x", 4, "float32", cands=cands, reps=1)
+    assert best["exp_rec"] == 0
+    with pytest.raises(ValueError):
+        autotune.tune(make, setup, "y", 4, "float32", cands=[{"exp_rec": 4}], reps=1)
