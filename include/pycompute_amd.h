@@ -289,6 +289,28 @@ int pca_mlp_colsum(const float* X, int64_t ld, int32_t M, int32_t N, float* out,
 int pca_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
                  float b2, float eps, int32_t step, float weight_decay, void* stream);
 
+/* One training step's forward, MSE gradient and backward in one kernel:
+ * workgroup k takes batch rows [64k, 64k + 64) and writes its partial
+ * gradient of every parameter block to part[k * np + ...] (entries outside
+ * the six blocks are left as they are) and its partial loss to loss_part[k].
+ * X (in_padded, ldx) feature-major, its column 0 the batch's first row; T
+ * (B, out) the batch's targets; offsets = the offsets of w1 (hidden,
+ * in_padded), b1, w2 (hidden, hidden), b2, w3 (out, hidden), b3 in the flat
+ * parameter vector w of np entries. B a positive multiple of 64; hidden 32 or
+ * 64 with in_padded 16 and out <= 16, or hidden 128 with in_padded 16, 32 or
+ * 64 and out <= 32. The same arithmetic as the GEMM path (bf16 operands, fp32
+ * accumulation); loss scale and dY = 2 (Y - T) / (B out). */
+int pca_mlp_train_step(const float* X, int64_t ldx, const float* T, int32_t B, const float* w,
+                       int32_t hidden, int32_t in_padded, int32_t out, const int32_t* offsets,
+                       int32_t np, float* part, float* loss_part, void* stream);
+
+/* g[j] = sum over k < nwg of part[k * np + j], in a fixed order;
+ * *loss += sum of loss_part (loss may be NULL). With w set, also one Adam
+ * step of w, m, v (as pca_mlp_adam) on g; with w NULL, gradients only. */
+int pca_mlp_grad_reduce(const float* part, const float* loss_part, int32_t nwg, int32_t np,
+                        float* g, float* loss, float* w, float* m, float* v, float lr, float b1,
+                        float b2, float eps, int32_t step, float weight_decay, void* stream);
+
 /* p[0:n] = value; fp32 -> bf16, fp32 -> fp16 (round to nearest even). */
 int pca_mlp_fill(float* p, int64_t n, float value, void* stream);
 int pca_mlp_to_bf16(const float* x, int16_t* y, int64_t n, void* stream);

@@ -261,17 +261,347 @@ __global__ void __launch_bounds__(256) k_colsum(const float* X, long long ld, in
   }
 }
 
-// Adam (Kingma and Ba 2015) with bias correction, on a flat parameter vector.
-__global__ void k_adam(float* w, const float* g, float* m, float* v, long long n, float lr,
-                       float b1, float b2, float eps, float c1, float c2, float wd) {
+// Adam (Kingma and Ba 2015) with bias correction c1 = 1 - b1^t, c2 = 1 - b2^t.
+struct AdamArgs {
+  float* w;
+  float* m;
+  float* v;
+  float lr, b1, b2, eps, c1, c2, wd;
+};
+
+__device__ __forceinline__ void adam_update(const AdamArgs& a, long long i, float g) {
+  const float gi = g + a.wd * a.w[i];
+  const float mi = a.b1 * a.m[i] + (1.f - a.b1) * gi;
+  const float vi = a.b2 * a.v[i] + (1.f - a.b2) * gi * gi;
+  a.m[i] = mi;
+  a.v[i] = vi;
+  a.w[i] -= a.lr * (mi / a.c1) / (sqrtf(vi / a.c2) + a.eps);
+}
+
+// Adam on a flat parameter vector.
+__global__ void k_adam(AdamArgs a, const float* g, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const float gi = g[i] + wd * w[i];
-    const float mi = b1 * m[i] + (1.f - b1) * gi;
-    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+       i += (long long)gridDim.x * blockDim.x)
+    adam_update(a, i, g[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused training step. One workgroup takes 64 rows of the batch through the
+// whole step -- forward (three layers), MSE gradient, backward -- with the
+// weights, its inputs and every activation and gradient matrix held in LDS as
+// bf16 (the MFMA operands; fp32 accumulation), and writes its partial
+// gradient of every parameter and its partial loss to one row of a
+// workspace. k_grad_reduce sums the rows in a fixed order (no atomics, so a
+// step is deterministic) and applies Adam to the sums.
+//
+// LDS arrays of activations and gradients are [feature][batch row] (row
+// length 64 + 4, the pad spreading 8-byte fragment reads over the banks), so
+// a C fragment (4 consecutive rows of one column per lane) is one 8-byte
+// store and the weight gradients (contractions over the batch) read both
+// operands 8 bytes at a time; the forward and backward-data products read
+// their batch-row operand with 4 two-byte reads. Weights keep their
+// [out][in] layout. Rounding is that of the GEMM path: bf16 operands (the
+// activations, weights, dY and the dZ), fp32 pre-activations, GELU
+// derivatives, dY, dZ and bias sums.
+// ---------------------------------------------------------------------------
+
+constexpr int TRB = 64;   // batch rows per workgroup
+
+struct TrainArgs {
+  const float* X;     // (in_padded, ldx) feature-major; column 0 = first row of the batch
+  const float* T;     // (B, O) targets of the batch
+  const float* w;     // flat parameters
+  float* part;        // (B / TRB, np) partial gradients (entries outside the blocks untouched)
+  float* loss_part;   // (B / TRB) partial losses
+  long long ldx;
+  int o1, ob1, o2, ob2, o3, ob3;   // offsets of w1 b1 w2 b2 w3 b3 in w
+  int np, O;
+  float scale;                     // 1 / (B O)
+};
+
+__device__ __forceinline__ bf16x4 lds_row4(const short* p) { return *(const bf16x4*)p; }
+__device__ __forceinline__ bf16x4 lds_col4(const short* p, int ld) {
+  return bf16x4{p[0], p[ld], p[2 * ld], p[3 * ld]};
+}
+
+// acc[j] += A B over one wave's 16 batch rows: A (16 x 16 KS) from a
+// [k][batch row] array (ld lda) at row row0; B (16 KS x 16 NT) from a
+// [col][k] array (BT false) or a [k][col] array (BT true), ld ldb.
+template <int NT, int KS, bool BT>
+__device__ __forceinline__ void wave_rows(floatx4* acc, const short* A, int lda, int row0,
+                                          const short* B, int ldb, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < KS; ++kk) {
+    const int k = 16 * kk + 4 * g;
+    const bf16x4 a = lds_col4(A + k * lda + row0 + r, lda);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int n = 16 * j + r;
+      const bf16x4 b = BT ? lds_col4(B + k * ldb + n, ldb) : lds_row4(B + n * ldb + k);
+      acc[j] = mfma16(a, b, acc[j]);
+    }
+  }
+}
+
+// One 16 x 16 tile of A^T B, contracted over the workgroup's batch rows: rows
+// ra.. of A and rb.. of B, both [feature][batch row] arrays.
+__device__ __forceinline__ floatx4 tile_tn(const short* A, int ra, const short* B, int rb,
+                                           int lane) {
+  constexpr int L = TRB + 4;
+  const int r = lane & 15, g = lane >> 4;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < TRB / 16; ++kk) {
+    const int k = 16 * kk + 4 * g;
+    acc = mfma16(lds_row4(A + (ra + r) * L + k), lds_row4(B + (rb + r) * L + k), acc);
+  }
+  return acc;
+}
+
+// A C fragment of tile (rows ra.., columns cb..) into the row-major p (row
+// stride ld); rows from rmax on are dropped.
+__device__ __forceinline__ void store_tile(float* p, int ld, int ra, int cb, int rmax, floatx4 v,
+                                           int lane) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = ra + 4 * g + q;
+    if (row < rmax) p[row * ld + cb + r] = v[q];
+  }
+}
+
+// bf16 of the 4 rows a lane holds, into a [column][batch row] array.
+__device__ __forceinline__ void put_rows(short* p, floatx4 v) {
+  *(bf16x4*)p = bf16x4{f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
+}
+
+// Sum over the 16 rows of a wave's C fragment: lanes 0..15 hold column r.
+__device__ __forceinline__ float col_sum(floatx4 v) {
+  float s = (v[0] + v[1]) + (v[2] + v[3]);
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s;
+}
+
+template <int H, int I, int OP>
+__global__ void __launch_bounds__(256) k_train_step(TrainArgs a) {
+  constexpr int L = TRB + 4, LI = I + 4, LH = H + 4;
+  constexpr int NH = H / 16, NI = I / 16, NO = OP / 16, NR = 2 * H + OP + 1;
+  __shared__ __attribute__((aligned(16))) short sX[I * L];     // X       [i][s]
+  __shared__ __attribute__((aligned(16))) short sW1[H * LI];   // w1      [h][i]
+  __shared__ __attribute__((aligned(16))) short sW2[H * LH];   // w2      [h'][h]
+  __shared__ __attribute__((aligned(16))) short sW3[OP * LH];  // w3      [o][h], zero rows >= O
+  __shared__ __attribute__((aligned(16))) short sH1[H * L];    // H1      [h][s]
+  __shared__ __attribute__((aligned(16))) short sH2[H * L];    // H2, then dZ1
+  __shared__ __attribute__((aligned(16))) short sZ2[H * L];    // dZ2
+  __shared__ __attribute__((aligned(16))) short sD[OP * L];    // dY      [o][s]
+  __shared__ float sred[4][NR];   // per wave: bias sums b1 | b2 | b3, loss
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, r = lane & 15, g = lane >> 4;
+  const long long s0 = (long long)blockIdx.x * TRB;
+  const int row0 = 16 * wv;   // the wave's batch rows in the row-tiled products
+
+  // staging: the trip counts are compile-time, so every thread's loads are in
+  // flight together before the first conversion waits
+  static_assert((H * I) % 256 == 0 && (H * H) % 256 == 0 && (OP * H) % 256 == 0 &&
+                    (I * TRB) % 256 == 0, "staging loops take whole rounds of the block");
+  {
+    float v1[H * I / 256], v2[H * H / 256], v3[OP * H / 256], vx[I * TRB / 256];
+#pragma unroll
+    for (int k = 0; k < H * I / 256; ++k) v1[k] = a.w[a.o1 + t + 256 * k];
+#pragma unroll
+    for (int k = 0; k < H * H / 256; ++k) v2[k] = a.w[a.o2 + t + 256 * k];
+#pragma unroll
+    for (int k = 0; k < OP * H / 256; ++k) {
+      const int e = t + 256 * k;
+      v3[k] = e / H < a.O ? a.w[a.o3 + e] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < I * TRB / 256; ++k) {
+      const int e = t + 256 * k;
+      vx[k] = a.X[(e / TRB) * a.ldx + s0 + e % TRB];
+    }
+#pragma unroll
+    for (int k = 0; k < H * I / 256; ++k) {
+      const int e = t + 256 * k;
+      sW1[(e / I) * LI + e % I] = f2bf(v1[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < H * H / 256; ++k) {
+      const int e = t + 256 * k;
+      sW2[(e / H) * LH + e % H] = f2bf(v2[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < OP * H / 256; ++k) {
+      const int e = t + 256 * k;
+      sW3[(e / H) * LH + e % H] = f2bf(v3[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < I * TRB / 256; ++k) {
+      const int e = t + 256 * k;
+      sX[(e / TRB) * L + e % TRB] = f2bf(vx[k]);
+    }
+  }
+  __syncthreads();
+
+  floatx4 acc[NH];
+  floatx4 gg1[NH], gg2[NH];   // gelu'(Z1), gelu'(Z2) of the wave's rows
+  // H1 = gelu(X w1^T + b1)
+#pragma unroll
+  for (int j = 0; j < NH; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  wave_rows<NH, NI, false>(acc, sX, L, row0, sW1, LI, lane);
+#pragma unroll
+  for (int j = 0; j < NH; ++j) {
+    const int h = 16 * j + r;
+    const float b = a.w[a.ob1 + h];
+    floatx4 hv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float z = acc[j][q] + b;
+      gg1[j][q] = gelu_grad(z);
+      hv[q] = gelu(z);
+    }
+    put_rows(sH1 + h * L + row0 + 4 * g, hv);
+  }
+  __syncthreads();
+  // H2 = gelu(H1 w2^T + b2)
+#pragma unroll
+  for (int j = 0; j < NH; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  wave_rows<NH, NH, false>(acc, sH1, L, row0, sW2, LH, lane);
+#pragma unroll
+  for (int j = 0; j < NH; ++j) {
+    const int h = 16 * j + r;
+    const float b = a.w[a.ob2 + h];
+    floatx4 hv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float z = acc[j][q] + b;
+      gg2[j][q] = gelu_grad(z);
+      hv[q] = gelu(z);
+    }
+    put_rows(sH2 + h * L + row0 + 4 * g, hv);
+  }
+  __syncthreads();
+  // Y = H2 w3^T + b3; dY = 2 (Y - T) / (B O); loss = sum (Y - T)^2 / (B O)
+  {
+    floatx4 y[NO];
+#pragma unroll
+    for (int j = 0; j < NO; ++j) y[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    wave_rows<NO, NH, false>(y, sH2, L, row0, sW3, LH, lane);
+    float ls = 0.f;
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+      const int o = 16 * j + r;
+      floatx4 d = {0.f, 0.f, 0.f, 0.f};
+      if (o < a.O) {
+        const float b = a.w[a.ob3 + o];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float df = y[j][q] + b - a.T[(s0 + row0 + 4 * g + q) * a.O + o];
+          ls += df * df * a.scale;
+          d[q] = 2.f * df * a.scale;
+        }
+      }
+      const float cs = col_sum(d);
+      if (g == 0) sred[wv][2 * H + o] = cs;
+      put_rows(sD + o * L + row0 + 4 * g, d);
+    }
+    for (int o = 32; o > 0; o >>= 1) ls += __shfl_xor(ls, o);
+    if (lane == 0) sred[wv][NR - 1] = ls;
+  }
+  __syncthreads();
+  float* P = a.part + (long long)blockIdx.x * a.np;
+  // dw3 = dY^T H2
+  for (int tt = wv; tt < NO * NH; tt += 4) {
+    const int ro = tt / NH, co = tt % NH;
+    store_tile(P + a.o3, H, 16 * ro, 16 * co, a.O, tile_tn(sD, 16 * ro, sH2, 16 * co, lane),
+               lane);
+  }
+  // dZ2 = (dY w3) gelu'(Z2)
+#pragma unroll
+  for (int j = 0; j < NH; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  wave_rows<NH, NO, true>(acc, sD, L, row0, sW3, LH, lane);
+#pragma unroll
+  for (int j = 0; j < NH; ++j) {
+    const int h = 16 * j + r;
+    const floatx4 dz = acc[j] * gg2[j];
+    const float cs = col_sum(dz);
+    if (g == 0) sred[wv][H + h] = cs;
+    put_rows(sZ2 + h * L + row0 + 4 * g, dz);
+  }
+  __syncthreads();
+  // dw2 = dZ2^T H1
+  for (int tt = wv; tt < NH * NH; tt += 4) {
+    const int ro = tt / NH, co = tt % NH;
+    store_tile(P + a.o2, H, 16 * ro, 16 * co, H, tile_tn(sZ2, 16 * ro, sH1, 16 * co, lane),
+               lane);
+  }
+  // dZ1 = (dZ2 w2) gelu'(Z1), into H2's array (last read by dw3, before the barrier)
+#pragma unroll
+  for (int j = 0; j < NH; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  wave_rows<NH, NH, true>(acc, sZ2, L, row0, sW2, LH, lane);
+#pragma unroll
+  for (int j = 0; j < NH; ++j) {
+    const int h = 16 * j + r;
+    const floatx4 dz = acc[j] * gg1[j];
+    const float cs = col_sum(dz);
+    if (g == 0) sred[wv][h] = cs;
+    put_rows(sH2 + h * L + row0 + 4 * g, dz);
+  }
+  __syncthreads();
+  // dw1 = dZ1^T X
+  for (int tt = wv; tt < NH * NI; tt += 4) {
+    const int ro = tt / NI, co = tt % NI;
+    store_tile(P + a.o1, I, 16 * ro, 16 * co, H, tile_tn(sH2, 16 * ro, sX, 16 * co, lane), lane);
+  }
+  for (int e = t; e < 2 * H + OP; e += 256) {
+    const float s = (sred[0][e] + sred[1][e]) + (sred[2][e] + sred[3][e]);
+    if (e < H)
+      P[a.ob1 + e] = s;
+    else if (e < 2 * H)
+      P[a.ob2 + e - H] = s;
+    else if (e - 2 * H < a.O)
+      P[a.ob3 + e - 2 * H] = s;
+  }
+  if (t == 0)
+    a.loss_part[blockIdx.x] =
+        (sred[0][NR - 1] + sred[1][NR - 1]) + (sred[2][NR - 1] + sred[3][NR - 1]);
+}
+
+// g[j] = sum over workspace rows k of part[k][j], in a fixed order: a block
+// takes 32 columns, its 8 thread groups sum rows q, q + 8, ... (rounds of 16
+// rows whose loads are all in flight together, into four interleaved partial
+// sums), and the 8 group sums meet in LDS. With ad.w set, Adam on the sums. Block 0 adds
+// the partial losses to *loss.
+__global__ void __launch_bounds__(256) k_grad_reduce(const float* part, const float* loss_part,
+                                                    int nwg, int np, float* g, float* loss,
+                                                    AdamArgs ad) {
+  __shared__ float red[8][32];
+  const int c = threadIdx.x & 31, q = threadIdx.x >> 5;
+  const int j = blockIdx.x * 32 + c;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  if (j < np)
+    for (int k0 = q; k0 < nwg; k0 += 128) {
+      float v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        v[u] = k0 + 8 * u < nwg ? part[(long long)(k0 + 8 * u) * np + j] : 0.f;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s[u & 3] += v[u];
+    }
+  red[q][c] = (s[0] + s[1]) + (s[2] + s[3]);
+  __syncthreads();
+  if (q == 0 && j < np) {
+    const float gj = ((red[0][c] + red[1][c]) + (red[2][c] + red[3][c])) +
+                     ((red[4][c] + red[5][c]) + (red[6][c] + red[7][c]));
+    g[j] = gj;
+    if (ad.w) adam_update(ad, j, gj);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64 && loss) {   // one wave, butterfly in a fixed order
+    float l = 0.f;
+    for (int k = threadIdx.x; k < nwg; k += 64) l += loss_part[k];
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+    if (threadIdx.x == 0) *loss += l;
   }
 }
 
@@ -576,6 +906,17 @@ __global__ void k_prepare(const float* y, const float* truth, long long N, int I
 
 int grid_1d(long long n) { return (int)std::min<long long>((n + 255) / 256, 4096); }
 
+template <int H, int I, int OP>
+void launch_train(const TrainArgs& a, int B, hipStream_t s) {
+  hipLaunchKernelGGL((k_train_step<H, I, OP>), dim3(B / TRB), dim3(256), 0, s, a);
+}
+
+AdamArgs adam_args(float* w, float* m, float* v, float lr, float b1, float b2, float eps, int step,
+                   float wd) {
+  return AdamArgs{w, m, v, lr, b1, b2, eps, 1.f - powf(b1, (float)step),
+                  1.f - powf(b2, (float)step), wd};
+}
+
 int check_launch() {
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? PCA_OK : pca_fail(PCA_ERR_HIP, hipGetErrorString(e));
@@ -627,9 +968,57 @@ extern "C" int pca_mlp_adam(float* w, const float* g, float* m, float* v, int64_
                             void* stream) {
   if (!w || !g || !m || !v || n <= 0 || step < 1)
     return pca_fail(PCA_ERR_ARG, "pca_mlp_adam: bad argument");
-  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
-  hipLaunchKernelGGL(k_adam, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v,
-                     (long long)n, lr, b1, b2, eps, c1, c2, weight_decay);
+  const AdamArgs ad = adam_args(w, m, v, lr, b1, b2, eps, step, weight_decay);
+  hipLaunchKernelGGL(k_adam, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, ad, g,
+                     (long long)n);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_train_step(const float* X, int64_t ldx, const float* T, int32_t B,
+                                  const float* w, int32_t hidden, int32_t in_padded, int32_t out,
+                                  const int32_t* offsets, int32_t np, float* part,
+                                  float* loss_part, void* stream) {
+  if (!X || !T || !w || !offsets || !part || !loss_part || B <= 0 || B % TRB != 0 || out < 1 ||
+      np <= 0 || ldx < B)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_train_step: bad argument (B a positive multiple of 64)");
+  const int OP = out <= 16 ? 16 : 32;
+  const bool narrow = hidden == 32 || hidden == 64;
+  if (!((narrow && in_padded == 16 && OP == 16) ||
+        (hidden == 128 && (in_padded == 16 || in_padded == 32 || in_padded == 64) && out <= 32)))
+    return pca_fail(PCA_ERR_ARG,
+                    "pca_mlp_train_step: hidden 32 or 64 (in_padded 16, out <= 16) or 128 "
+                    "(in_padded 16, 32 or 64, out <= 32)");
+  const long long sizes[6] = {(long long)hidden * in_padded, hidden, (long long)hidden * hidden,
+                              hidden, (long long)out * hidden, out};
+  for (int k = 0; k < 6; ++k)
+    if (offsets[k] < 0 || offsets[k] + sizes[k] > np)
+      return pca_fail(PCA_ERR_ARG, "pca_mlp_train_step: parameter block outside np");
+  TrainArgs a{X, T, w, part, loss_part, (long long)ldx, offsets[0], offsets[1], offsets[2],
+              offsets[3], offsets[4], offsets[5], np, out, 1.f / ((float)B * (float)out)};
+  const hipStream_t s = (hipStream_t)stream;
+  if (hidden == 32)
+    launch_train<32, 16, 16>(a, B, s);
+  else if (hidden == 64)
+    launch_train<64, 16, 16>(a, B, s);
+  else if (in_padded == 16)
+    OP == 16 ? launch_train<128, 16, 16>(a, B, s) : launch_train<128, 16, 32>(a, B, s);
+  else if (in_padded == 32)
+    OP == 16 ? launch_train<128, 32, 16>(a, B, s) : launch_train<128, 32, 32>(a, B, s);
+  else
+    OP == 16 ? launch_train<128, 64, 16>(a, B, s) : launch_train<128, 64, 32>(a, B, s);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_grad_reduce(const float* part, const float* loss_part, int32_t nwg,
+                                   int32_t np, float* g, float* loss, float* w, float* m, float* v,
+                                   float lr, float b1, float b2, float eps, int32_t step,
+                                   float weight_decay, void* stream) {
+  if (!part || !loss_part || !g || nwg <= 0 || np <= 0 || (w && (!m || !v || step < 1)))
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_grad_reduce: bad argument");
+  const AdamArgs ad = w ? adam_args(w, m, v, lr, b1, b2, eps, step, weight_decay)
+                        : AdamArgs{nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0.f};
+  hipLaunchKernelGGL(k_grad_reduce, dim3((np + 31) / 32), dim3(256), 0, (hipStream_t)stream, part,
+                     loss_part, nwg, np, g, loss, ad);
   return check_launch();
 }
 

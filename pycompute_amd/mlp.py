@@ -61,9 +61,13 @@ def _bind():
     L.pca_mlp_prepare.argtypes = [_P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P,
                                   _P, _P, _P]
     L.pca_mlp_infer.argtypes = [ctypes.POINTER(InferArgs), _P]
+    L.pca_mlp_train_step.argtypes = [_P, _I64, _P, _I32, _P, _I32, _I32, _I32,
+                                     ctypes.POINTER(_I32), _I32, _P, _P, _P]
+    L.pca_mlp_grad_reduce.argtypes = [_P, _P, _I32, _I32, _P, _P, _P, _P, _P, _F, _F, _F, _F,
+                                      _I32, _F, _P]
     for n in ("pca_mlp_gemm", "pca_mlp_mse_grad", "pca_mlp_colsum", "pca_mlp_adam",
               "pca_mlp_fill", "pca_mlp_to_bf16", "pca_mlp_to_f16", "pca_mlp_prepare",
-              "pca_mlp_infer"):
+              "pca_mlp_infer", "pca_mlp_train_step", "pca_mlp_grad_reduce"):
         getattr(L, n).restype = ctypes.c_int
     L._mlp_bound = True
     return L
@@ -260,6 +264,72 @@ class MLPInit:
         _lib.check(L.pca_mlp_colsum(b["dZ1"].data_ptr(), H, B, H, g["b1"].data_ptr(), _stream()),
                    "colsum")
 
+    # -- fused training step ------------------------------------------------------
+    FUSED_ROWS = 64   # batch rows per workgroup of pca_mlp_train_step
+
+    def fused_supported(self, B: int) -> bool:
+        """pca_mlp_train_step covers hidden 32 and 64 (16 padded inputs, <= 16
+        outputs) and 128 (16, 32 or 64 padded inputs, <= 32 outputs), batches
+        of a multiple of 64 rows; wider networks train on the GEMM path."""
+        if B <= 0 or B % self.FUSED_ROWS:
+            return False
+        if self.H in (32, 64):
+            return self.INP == 16 and self.OUT <= 16
+        return self.H == 128
+
+    def _fused_ws(self, B):
+        nwg = B // self.FUSED_ROWS
+        ws = getattr(self, "_ws", None)
+        if ws is None or ws["nwg"] != nwg:
+            kw = dict(dtype=torch.float32, device=self.device)
+            # zeros once: entries between the parameter blocks are never written
+            ws = {"nwg": nwg, "part": torch.zeros((nwg, self.nparams), **kw),
+                  "loss_part": torch.zeros(nwg, **kw), "loss": torch.zeros(1, **kw),
+                  "offsets": (_I32 * 6)(*[self.offsets[k][0]
+                                          for k in ("w1", "b1", "w2", "b2", "w3", "b3")])}
+            self._ws = ws
+        return ws
+
+    def fused_step(self, X, T, off: int, B: int, ldx: int, lr=None, b1=0.9, b2=0.999, eps=1e-8,
+                   weight_decay=0.0):
+        """One training step on the batch [off, off+B) in two kernels:
+        pca_mlp_train_step (forward, loss gradient and backward, one
+        workgroup per 64 rows, partial gradients to a workspace) and
+        pca_mlp_grad_reduce (the sums into self.g, the loss into the fused
+        loss buffer, and with lr given one Adam step). With lr None the
+        weights are left as they are."""
+        if not self.fused_supported(B):
+            raise ValueError(f"fused training step: hidden {self.H}, {self.INP} padded inputs, "
+                             f"{self.OUT} outputs, batch {B} not covered")
+        if off < 0 or off + B > ldx or tuple(X.shape) != (self.INP, ldx) \
+                or T.shape[0] < off + B or T.shape[1] != self.OUT:
+            raise ValueError("fused training step: X (in_padded, ldx), T (>= off + B, out) "
+                             "and off + B <= ldx")
+        if not (X.is_contiguous() and T.is_contiguous() and X.dtype == T.dtype == torch.float32):
+            raise ValueError("fused training step: contiguous float32 X and T")
+        ws = self._fused_ws(B)
+        self._fused_launch(ws, X.data_ptr() + 4 * off, T.data_ptr() + 4 * off * self.OUT, B, ldx,
+                           lr, b1, b2, eps, weight_decay)
+
+    def _fused_launch(self, ws, xp, tp, B, ldx, lr, b1=0.9, b2=0.999, eps=1e-8,
+                      weight_decay=0.0):
+        L = _bind()
+        st = _stream()
+        _lib.check(L.pca_mlp_train_step(xp, ldx, tp, B, self.w.data_ptr(), self.H, self.INP,
+                                        self.OUT, ws["offsets"], self.nparams,
+                                        ws["part"].data_ptr(), ws["loss_part"].data_ptr(), st),
+                   "pca_mlp_train_step")
+        if lr is None:
+            wp = mp = vp = None
+            step = 0
+        else:
+            self.step += 1
+            wp, mp, vp, step = self.w.data_ptr(), self.m.data_ptr(), self.v.data_ptr(), self.step
+        _lib.check(L.pca_mlp_grad_reduce(ws["part"].data_ptr(), ws["loss_part"].data_ptr(),
+                                         ws["nwg"], self.nparams, self.g.data_ptr(),
+                                         ws["loss"].data_ptr(), wp, mp, vp, lr or 0.0, b1, b2,
+                                         eps, step, weight_decay, st), "pca_mlp_grad_reduce")
+
     def adam_step(self, lr, b1=0.9, b2=0.999, eps=1e-8, weight_decay=0.0):
         L = _bind()
         self.step += 1
@@ -269,19 +339,33 @@ class MLPInit:
                                   weight_decay, _stream()), "pca_mlp_adam")
 
     def train(self, X, T, epochs: int = 2, batch: int = 8192, lr: float = 2e-3,
-              lr_min: float = 2e-5, log=None):
+              lr_min: float = 2e-5, log=None, fused=None):
+        """Adam with a cosine learning-rate schedule over epochs x (n // batch)
+        steps. fused None: the fused step where it covers the network and the
+        batch, else the GEMM path; True or False forces one."""
         n = X.shape[1]
         steps_per_epoch = n // batch
         total = max(1, epochs * steps_per_epoch)
-        b = self._buffers(batch)
+        if fused is None:
+            fused = self.fused_supported(batch)
+        loss_t = self._fused_ws(batch)["loss"] if fused else self._buffers(batch)["loss"]
         for e in range(epochs):
-            b["loss"].zero_()
+            loss_t.zero_()
             for s in range(steps_per_epoch):
                 k = e * steps_per_epoch + s
                 lr_k = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / total))
-                self.forward_backward(X, T, s * batch, batch, n)
-                self.adam_step(lr_k)
-            loss = float(b["loss"].item()) / max(1, steps_per_epoch)
+                if fused:
+                    if k == 0:   # argument checks once; the loop launches directly
+                        self.fused_step(X, T, 0, batch, n, lr=lr_k)
+                        ws = self._fused_ws(batch)
+                        xp, tp = X.data_ptr(), T.data_ptr()
+                    else:
+                        self._fused_launch(ws, xp + 4 * s * batch,
+                                           tp + 4 * s * batch * self.OUT, batch, n, lr_k)
+                else:
+                    self.forward_backward(X, T, s * batch, batch, n)
+                    self.adam_step(lr_k)
+            loss = float(loss_t.item()) / max(1, steps_per_epoch)
             self.history.append(loss)
             if log:
                 log(f"mlp epoch {e}: mean loss {loss:.5f}")

@@ -90,6 +90,29 @@ def test_mlp_infer_args_layout_and_validation():
     assert b"NULL" in L.pca_last_error()
 
 
+def test_mlp_train_step_validation():
+    """The fused training step and the gradient reduction reject what their
+    kernels do not cover before any launch."""
+    from pycompute_amd import mlp
+    L = mlp._bind()
+    offs = (ctypes.c_int32 * 6)(0, 1024, 1088, 5184, 5248, 5504)   # hidden 64, 16 in, 4 out
+    np_ = 5512
+
+    def step(B=8192, hidden=64, inp=16, out=4, n=np_, ldx=8192):
+        return L.pca_mlp_train_step(1, ldx, 1, B, 1, hidden, inp, out, offs, n, 1, 1, None)
+
+    for kw, msg in (({"B": 100}, b"multiple of 64"), ({"hidden": 256}, b"hidden"),
+                    ({"inp": 32}, b"hidden"), ({"out": 17}, b"hidden"),
+                    ({"n": 5000}, b"outside np"), ({"ldx": 64}, b"bad argument")):
+        assert step(**kw) == -1, kw
+        assert msg in L.pca_last_error(), (kw, L.pca_last_error())
+    assert L.pca_mlp_grad_reduce(1, 1, 0, np_, 1, None, None, None, None, 1e-3, 0.9, 0.999,
+                                 1e-8, 1, 0.0, None) == -1
+    assert L.pca_mlp_grad_reduce(1, 1, 128, np_, 1, None, 1, 1, 1, 1e-3, 0.9, 0.999, 1e-8, 0,
+                                 0.0, None) == -1          # Adam needs step >= 1
+    assert b"pca_mlp_grad_reduce" in L.pca_last_error()
+
+
 def test_zero_voxel_calls_accept_empty_arrays():
     """With no voxels the per-voxel arrays of an empty tensor have no address;
     the entry points return PCA_OK without launching (no GPU needed)."""

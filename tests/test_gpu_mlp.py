@@ -103,6 +103,96 @@ def test_training_gradients_match_oracle(gpu):
         assert rel < 2e-3, (k, rel)
 
 
+def _fused_case(case):
+    if case == "dki128":
+        return mlp.dki_mlp(synth.dki(16, seed=1)["consts"], hidden=128, seed=3)
+    return mlp.ivim_mlp(synth.B_C2, hidden={"ivim32": 32, "ivim64": 64, "ivim128": 128}[case],
+                        seed=4)
+
+
+@pytest.mark.parametrize("case", ["ivim32", "ivim64", "ivim128", "dki128"])
+def test_fused_step_gradients_match_oracle(gpu, case):
+    """pca_mlp_train_step + pca_mlp_grad_reduce (no Adam): loss and every
+    parameter block's gradient against the float64 oracle with bf16 operands,
+    and against the GEMM path; 640 batch rows (10 workgroups), inputs read from
+    a column offset of a wider X. dki128: 61 inputs (64 padded), 22 outputs."""
+    net = _fused_case(case)
+    rng = np.random.default_rng(5)
+    B, off, n = 640, 192, 1024
+    X = rng.uniform(0.1, 1.2, (net.INP, n)).astype(np.float32)
+    X[net.IN:] = 0
+    T = rng.standard_normal((n, net.OUT)).astype(np.float32)
+    Xt, Tt = torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda")
+    print(f"[fused step {case}] H={net.H} in={net.INP} out={net.OUT} B={B}", flush=True)
+    net._fused_ws(B)["loss"].zero_()
+    w0 = net.w.clone()
+    net.fused_step(Xt, Tt, off, B, n)
+    torch.cuda.synchronize()
+    assert torch.equal(net.w, w0)                     # no Adam without lr
+    fused_loss = net._fused_ws(B)["loss"].item()
+    fused_g = net.g.clone()
+    net._buffers(B)["loss"].zero_()
+    net.forward_backward(Xt, Tt, off, B, n)
+    torch.cuda.synchronize()
+    w = {k: net._view(net.w, k).double().cpu().numpy() for k in net.offsets}
+    loss, g, _ = omlp.forward_backward(X[:, off:off + B].T.astype(np.float64),
+                                       T[off:off + B].astype(np.float64), w["w1"], w["b1"],
+                                       w["w2"], w["b2"], w["w3"], w["b3"])
+    assert abs(fused_loss - loss) / loss < 1e-4, (fused_loss, loss)
+    for k in net.offsets:
+        got = net._view(fused_g, k).double().cpu().numpy()
+        gemm = net._view(net.g, k).double().cpu().numpy()
+        rel = np.linalg.norm(got - g[k]) / np.linalg.norm(g[k])
+        rel_gemm = np.linalg.norm(got - gemm) / np.linalg.norm(g[k])
+        print(f"[fused step {case}] grad {k}: vs oracle {rel:.2e}, vs GEMM path {rel_gemm:.2e}",
+              flush=True)
+        assert rel < 2e-3, (k, rel)
+        assert rel_gemm < 2e-3, (k, rel_gemm)
+    pad = net._view(fused_g, "w1")[:, net.IN:]
+    assert torch.all(pad == 0)                        # padded inputs carry no gradient
+
+
+def test_fused_step_adam_equals_adam_kernel(gpu):
+    """The Adam update inside pca_mlp_grad_reduce equals pca_mlp_adam applied
+    to the same reduced gradient, bit for bit, over three steps; the reduction
+    is deterministic (the same batch twice gives the same gradient bits)."""
+    net = _fused_case("ivim64")
+    ref = _fused_case("ivim64")
+    rng = np.random.default_rng(6)
+    B = 1024
+    X = torch.tensor(rng.uniform(0.1, 1.2, (net.INP, B)).astype(np.float32), device="cuda")
+    X[net.IN:] = 0
+    T = torch.tensor(rng.standard_normal((B, net.OUT)).astype(np.float32), device="cuda")
+    for it in range(3):
+        lr = 1e-3 * (it + 1)
+        ref.fused_step(X, T, 0, B, B)                 # gradient only
+        g1 = ref.g.clone()
+        ref.fused_step(X, T, 0, B, B)
+        assert torch.equal(ref.g, g1)
+        ref.adam_step(lr)
+        net.fused_step(X, T, 0, B, B, lr=lr)
+        torch.cuda.synchronize()
+        assert torch.equal(net.g, ref.g)
+        for k in ("w", "m", "v"):
+            assert torch.equal(getattr(net, k), getattr(ref, k)), (it, k)
+
+
+def test_fused_training_matches_gemm_training(gpu):
+    """Two epochs on 1M forward-model signals: the fused step and the GEMM
+    path reach the same loss (within 2%) from the same initial weights."""
+    hist = {}
+    for fused in (True, False):
+        net = mlp.ivim_mlp(synth.B_C2, seed=7)
+        data, truth = net.synthesize(1 << 20, seed=99, snr=(20.0, 100.0),
+                                     lo=[0.8, 0.05, 5e-3, 0.5e-3], hi=[1.2, 0.4, 0.1, 3e-3])
+        net.fit_standardization(truth, data)
+        X, T = net.prepare(data, truth)
+        hist[fused] = net.train(X, T, epochs=2, batch=8192, fused=fused)
+    print(f"loss history fused {hist[True]}, GEMM path {hist[False]}", flush=True)
+    assert hist[True][-1] < hist[True][0]
+    assert abs(hist[True][-1] - hist[False][-1]) / hist[False][-1] < 0.02
+
+
 def test_fused_inference_matches_oracle_forward(gpu):
     net = _small_net(seed=1)
     w = synth.ivim_c2(3000, seed=9)
