@@ -248,15 +248,18 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
             "ivim", w["consts"], w["data"][:, :ns], prob.lb_t[:, :ns].cpu().numpy(),
             prob.ub_t[:, :ns].cpu().numpy(), g[:, :ns].cpu().numpy(), cap,
             solver.pars_t[:, :ns].cpu().numpy())
-    # component times: MLP inference alone, the mean of 20 back-to-back launches
+    # component times: MLP inference alone, the mean of 50 back-to-back launches
+    # after 30 untimed ones (the CPU oracle above leaves the GPU idle, and a
+    # cold start reads up to 1.5x slower)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, fallback=(prob.model, prob.consts))
+    for _ in range(30):
+        net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, fallback=(prob.model, prob.consts))
     e0.record(stream)
-    for _ in range(20):
+    for _ in range(50):
         net.infer(prob.data_t, prob.lb_t, prob.ub_t, pars, fallback=(prob.model, prob.consts))
     e1.record(stream)
     torch.cuda.synchronize()
-    infer_ms = e0.elapsed_time(e1) / 20
+    infer_ms = e0.elapsed_time(e1) / 50
     H = net.H
     mlp_flops = 2.0 * N * (16 * H + H * H + H * 4)
     # LM kernels of the MLP-initialised fit alone

@@ -670,7 +670,8 @@ struct InferArgs {
 // weight matrices (the 128-unit networks), 2 = W2 only (the 256-unit ones,
 // whose 128 KB W2 takes a workgroup of its own per CU); 0 = none.
 template <int H, int INP, int OUTP, int VT, int LDSW, int OUTC>
-__global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
+__global__ void __launch_bounds__(256, H == 64 && LDSW == 1 && OUTC > 0 ? 3 : 1)
+    k_mlp_infer_t(InferArgs a) {
   // outputs: OUTC when fixed at compile time (the per-output guards fold away), else a.OUT
   const int nout = OUTC > 0 ? OUTC : a.OUT;
   constexpr int UT = H / 16;     // unit tiles per hidden layer
@@ -682,8 +683,14 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   constexpr int W2OFF = H * INP, W3OFF = H * INP + H * H;
   constexpr int W2L = LDSW == 1 ? W2OFF : 0;   // offset of W2 in wl
   constexpr int WLN = LDSW == 1 ? W3OFF + OUTP * H : (LDSW == 2 ? H * H : 8);
+  constexpr bool PF = (H == 64 || H == 128) && INP <= 32;
   __shared__ float ybuf[4][OUTP * V];
   __shared__ __attribute__((aligned(16))) short wl[WLN];
+  // with the weights in LDS the biases go there too: b1 | b2 | b3 (zero past
+  // the outputs), read with ds_read, so no global load inside the voxel loop
+  // but the (prefetched) inputs, the bounds and the stores
+  constexpr int BLN = LDSW == 1 ? 2 * H + OUTP : 4;
+  __shared__ __attribute__((aligned(16))) float bl[BLN];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int col = lane & 15;      // voxel within a tile
@@ -691,6 +698,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   if constexpr (LDSW == 1) {
     stage_fragments<H, INP>(wl, a.w1, H);
     stage_fragments_k32<OUTP, H>(wl + W3OFF, a.w3, nout);
+    for (int i = threadIdx.x; i < BLN; i += blockDim.x)
+      bl[i] = i < H ? a.b1[i] : (i < 2 * H ? a.b2[i - H] : (i - 2 * H < nout ? a.b3[i - 2 * H] : 0.f));
   }
   if constexpr (LDSW != 0) {
     stage_fragments_k32<H, H>(wl + W2L, a.w2, H);
@@ -723,31 +732,59 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
   };
   float* yb = ybuf[wave];
   const long long nwaves = (long long)gridDim.x * 4;
-  for (long long base = ((long long)blockIdx.x * 4 + wave) * V; base < a.N; base += nwaves * V) {
-    // layer-1 B operands: normalised inputs x^T (k = feature, n = voxel); the
-    // normaliser of tile t stays in sv[t] for the epilogue
-    half4 xb[KT1][VT];
-    float sv[VT];
+  // raw inputs of one pass: measurement rows kq..kq+3 of each k tile and the
+  // normaliser, for voxel base + 16 t + col
+  auto load_x = [&](long long b, float (&xr)[VT][KT1][4], float (&sr)[VT]) {
 #pragma unroll
     for (int t = 0; t < VT; ++t) {
-      const long long v = base + t * 16 + col;
+      const long long v = b + t * 16 + col;
       const bool ok = v < a.N;
-      float s = ok ? a.data[(long long)a.norm_index * a.N + v] : 1.f;
-      float xr[KT1][4];
+      sr[t] = ok ? a.data[(long long)a.norm_index * a.N + v] : 1.f;
 #pragma unroll
       for (int kt = 0; kt < KT1; ++kt)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = kt * 16 + kq + q;
-          xr[kt][q] = (ok && i < a.IN) ? a.data[(long long)i * a.N + v] : 0.f;
+          xr[t][kt][q] = (ok && i < a.IN) ? a.data[(long long)i * a.N + v] : 0.f;
         }
+    }
+  };
+  const long long first = ((long long)blockIdx.x * 4 + wave) * V, stride = nwaves * V;
+  // PF: the next pass's inputs are loaded before this pass's layers run, so
+  // their latency hides under them (widths 64 and 128 with up to 32 inputs,
+  // whose register budgets keep their occupancy, 2 and 1 waves per SIMD, and
+  // spill nothing with the copy)
+  float xn[VT][KT1][4], sn[VT];
+  if (PF && first < a.N) load_x(first, xn, sn);
+  for (long long base = first; base < a.N; base += stride) {
+    // layer-1 B operands: normalised inputs x^T (k = feature, n = voxel); the
+    // normaliser of tile t stays in sv[t] for the epilogue
+    float xr[VT][KT1][4], sr[VT];
+    if constexpr (PF) {
+#pragma unroll
+      for (int t = 0; t < VT; ++t) {
+        sr[t] = sn[t];
+#pragma unroll
+        for (int kt = 0; kt < KT1; ++kt)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) xr[t][kt][q] = xn[t][kt][q];
+      }
+      if (base + stride < a.N) load_x(base + stride, xn, sn);
+    } else {
+      load_x(base, xr, sr);
+    }
+    half4 xb[KT1][VT];
+    float sv[VT];
+#pragma unroll
+    for (int t = 0; t < VT; ++t) {
+      float s = sr[t];
       if (!(s > 0.f)) s = 1.f;
       sv[t] = s;
       const float inv = 1.f / s;
 #pragma unroll
       for (int kt = 0; kt < KT1; ++kt)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) xb[kt][t][q] = (_Float16)(xr[kt][q] * inv);
+        for (int q = 0; q < 4; ++q) xb[kt][t][q] = (_Float16)(xr[t][kt][q] * inv);
     }
     // epilogue operands, loaded now so their latency hides under the layers:
     // lane l finishes voxel base + l
@@ -770,7 +807,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
     for (int u = 0; u < UT; ++u) {
       floatx4 bias;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) bias[q] = a.b1[u * 16 + kq + q];
+      for (int q = 0; q < 4; ++q) bias[q] = LDSW == 1 ? bl[u * 16 + kq + q] : a.b1[u * 16 + kq + q];
       floatx4 acc[VT];
 #pragma unroll
       for (int t = 0; t < VT; ++t) acc[t] = bias;
@@ -793,7 +830,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int j = o * 16 + kq + q;
-        b3v[q] = j < nout ? a.b3[j] : 0.f;
+        b3v[q] = LDSW == 1 ? bl[2 * H + j] : (j < nout ? a.b3[j] : 0.f);
       }
 #pragma unroll
       for (int t = 0; t < VT; ++t) acc3[o][t] = b3v;
@@ -808,7 +845,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer_t(InferArgs a) {
       for (int h = 0; h < 2; ++h) {
         floatx4 bias;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) bias[q] = a.b2[(2 * c + h) * 16 + kq + q];
+        for (int q = 0; q < 4; ++q)
+          bias[q] = LDSW == 1 ? bl[H + (2 * c + h) * 16 + kq + q] : a.b2[(2 * c + h) * 16 + kq + q];
 #pragma unroll
         for (int t = 0; t < VT; ++t) acc[h][t] = bias;
       }
@@ -1077,10 +1115,14 @@ extern "C" int pca_mlp_infer(const pca_mlp_infer_args* args, void* stream) {
   const int V = big ? 32 : 64;  // voxels per wavefront
   const long long waves = (args->N + V - 1) / V;
   // weights in LDS when they fit next to the output buffer in 64 KB: then a
-  // persistent grid of 512 workgroups (2 per CU) reuses the staged copy
+  // persistent grid, as many workgroups per CU as the registers let run at
+  // once (width 32: 4 waves per SIMD, 64: 3 with the output count fixed at
+  // compile time, 128: 2), reuses the staged copy
   const bool ldsw = !big && OUTP == 16 && INP <= 32;
+  const int per_cu = H == 32 ? 4 : (H == 64 && OUT <= 4 && OUT >= 2 ? 3 : 2);
   // 256-unit networks stage W2 (128 KB) instead: one persistent workgroup per CU
-  const int grid = (int)std::min<long long>((waves + 3) / 4, ldsw ? 512 : (big ? 256 : 8192));
+  const int grid =
+      (int)std::min<long long>((waves + 3) / 4, ldsw ? 256 * per_cu : (big ? 256 : 8192));
   hipStream_t s = (hipStream_t)stream;
 #define PCA_INFER(HH, II, OO, VT, L, OC) \
   hipLaunchKernelGGL((k_mlp_infer_t<HH, II, OO, VT, L, OC>), dim3(grid), dim3(256), 0, s, a)
