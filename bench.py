@@ -211,8 +211,15 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
     log(f"default guess: validated cap {dcap}, {dms:.3f} ms/step")
     del dsolver
 
+    # the capped fit's own build: a capped launch from the network's guess is
+    # one chunk of `cap` iterations, where the knob sets rank differently from
+    # the 100-iteration plain fit (the plain fit's choice is the reference
+    # candidate of this tuning, and its parity bar)
+    cap_knobs = _tune_capped(args, prob, net, knobs, cap)
+    log(f"capped fit tuned: {cap_knobs}")
+
     # the benchmark volume, one whole MLP-initialised fit per step
-    solver = fit.make_solver(prob, compaction=True, **knobs)
+    solver = fit.make_solver(prob, compaction=True, **cap_knobs)
     pars = torch.empty((4, N), dtype=torch.float32, device=prob.data_t.device)
     solver.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
 
@@ -285,6 +292,7 @@ def run_mlp(args, prob, knobs, plain_ssr, stream, plain_f=None, plain_ms=None, w
             "iterations_per_voxel": it, "total_final_ssr": ssr,
             "total_ssr_rel_vs_plain": ssr / plain_ssr - 1.0,
             "matched_total_residual": bool(ssr <= plain_ssr * (1 + 1e-4)),
+            "capped_fit_tuning": cap_knobs,
             "train_samples": args.train_samples, "train_epochs": args.train_epochs,
             "train_seconds": train_s, "train_loss": hist,
             "inference_ms": infer_ms,
@@ -320,6 +328,46 @@ def _train_ivim_net(args, hidden):
     log(f"mlp (width {hidden}) trained on {args.train_samples} signals in {train_s:.2f} s, "
         f"loss {hist}")
     return net, train_s, hist
+
+
+def _tune_capped(args, prob, net, knobs, cap):
+    """Knobs for the MLP-initialised fit capped at ``cap`` iterations: the
+    plain fit's knob set, then the fenced recurrence builds (anchors every 4
+    or 8 measurements, a fence every 1 or 2), timed on the capped fit from the
+    network's guess. --tuning fixes the knobs (with --mlp-tuning for this leg)."""
+    import torch
+    from pycompute_amd import autotune, fit
+    if args.mlp_tuning:
+        return dict(knobs, **json.loads(args.mlp_tuning))
+    if args.tuning:
+        return dict(knobs)
+    N = prob.N
+    dev = prob.data_t.device
+    g0 = torch.empty((prob.model.nparam, N), dtype=torch.float32, device=dev)
+    net.infer(prob.data_t, prob.lb_t, prob.ub_t, g0, fallback=(prob.model, prob.consts))
+    base = {k: v for k, v in knobs.items() if k != "keep_workspace"}
+    cands = [base]
+    if not args.reference_rounding:
+        for rec in (4, 8):
+            for sg in (1, 2):
+                c = dict(base, fast_exp=True, fast_div=True, reassoc=True, factored=True,
+                         waves_per_eu=5, sched_group=sg, exp_rec=rec, uc_lds=True, vpl=1)
+                if c not in cands:
+                    cands.append(c)
+
+    def restore(s):
+        s.pars_t.copy_(g0)
+        s.reset_state()
+
+    def make(**kn):
+        s = fit.make_solver(prob, **dict({"keep_workspace": False}, **kn))
+        s.setup(g0.clone(), prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
+        return s
+
+    kn = autotune.tune(make, restore, f"{prob.model.expr}|mlp-capped-{cap}", prob.M, "float32",
+                       allow_fast=not args.reference_rounding, iters=cap, reps=5, log=log,
+                       cands=cands)
+    return dict(kn, keep_workspace=False)
 
 
 def _validated_cap(net, pv, ref_ssr, knobs):
@@ -715,6 +763,8 @@ def main():
                     help="hidden width of the 1M-voxel fit's IVIM initialiser (the "
                          "4M-voxel volume uses the configuration's width 128)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-model batch")
+    ap.add_argument("--mlp-tuning", type=str, default=None,
+                    help="JSON knobs for the MLP-initialised capped fit (over --tuning)")
     ap.add_argument("--tuning", type=str, default=None,
                     help="JSON knob set for the IVIM fit instead of the autotuner's choice")
     ap.add_argument("--calibrate-copy", action="store_true",
