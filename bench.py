@@ -526,8 +526,12 @@ def run_ivim_4m(args, knobs, net, cap, val):
            "train_seconds": train_s}
     finals = {}
     ns = 8192
+    # the capped fit's build as tuned for the 1M volume (the same model, cap
+    # and protocol: the tuning cache answers), else tuned here
+    cap_knobs = _tune_capped(args, prob, net, knobs, cap)
+    out["capped_fit_tuning"] = cap_knobs
     for name, iters, use_mlp in (("plain_lm", 100, False), ("mlp_lm", cap, True)):
-        s = fit.make_solver(prob, compaction=True, **knobs)
+        s = fit.make_solver(prob, compaction=True, **(cap_knobs if use_mlp else knobs))
         pars = torch.empty((4, N), dtype=torch.float32, device=prob.data_t.device)
         s.setup(pars, prob.consts_t, prob.data_t, prob.lb_t, prob.ub_t)
 
