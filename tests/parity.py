@@ -18,7 +18,9 @@ same inputs by the FP32 and FP64 oracles:
     ill-conditioned model where the FP32 oracle itself agrees with the FP64
     one on fewer, on that fraction less two points);
   * parameters within 1e-4 of the FP64 optimum on at least as many voxels,
-    less two points, as the FP32 oracle's own parameters are;
+    less two points, as the FP32 oracle's own parameters are (less three
+    standard deviations of the paired difference, sqrt(discordant / n), when
+    that is more: a small sample of capped, unconverged fits);
   * residual tail: for t in (1e-4, 1e-3, 1e-2, 1e-1), the fraction of voxels
     whose final SSR exceeds the FP64 oracle's by more than t (relative) is at
     most twice the FP32 oracle's fraction plus 0.03% and two voxels, and the
@@ -82,8 +84,16 @@ def check(tag, model, consts, y, gpu_params, r32, r64, gpu_first_f=None, min_agr
     def prel(a, b):
         return (np.abs(a - b) / np.maximum(np.abs(b), 1e-7)).max(axis=0)
 
-    par_g = float(np.mean(prel(pg, r64["params"]) < 1e-4))
-    par_32 = float(np.mean(prel(np.asarray(r32["params"], d), r64["params"]) < 1e-4))
+    in_g = prel(pg, r64["params"]) < 1e-4
+    in_32 = prel(np.asarray(r32["params"], d), r64["params"]) < 1e-4
+    par_g, par_32 = float(np.mean(in_g)), float(np.mean(in_32))
+    # the two fractions are paired (same voxels): between two roundings that
+    # are equally good their difference has standard deviation about
+    # sqrt(discordant / n), discordant = voxels where exactly one of the two
+    # is within 1e-4 of FP64; the bar allows three of those, and at least two
+    # points (a capped fit of 2,048 voxels has 10-15% discordant voxels)
+    disc = float(np.mean(in_g != in_32))
+    par_slack = max(0.02, 3.0 * float(np.sqrt(disc / in_g.size)))
     eg = (Sg - S64) / np.abs(S64)
     e32 = (S32 - S64) / np.abs(S64)
     out = {"ssr_within_1e-4_of_fp32_oracle": agree, "params_within_1e-4_of_fp64": par_g,
@@ -92,7 +102,8 @@ def check(tag, model, consts, y, gpu_params, r32, r64, gpu_first_f=None, min_agr
            "fp32_oracle_excess_over_fp64": dict(zip(("p50", "p99", "p99.9", "max"),
                                                     map(float, _q(e32))))}
     print(f"[{tag}] N={pg.shape[1]} SSR within 1e-4 of FP32 oracle {agree:.5f}; params within "
-          f"1e-4 of FP64: gpu {par_g:.4f}, FP32 oracle {par_32:.4f}", flush=True)
+          f"1e-4 of FP64: gpu {par_g:.4f}, FP32 oracle {par_32:.4f} (discordant {disc:.4f}, bar "
+          f"{par_32 - par_slack:.4f})", flush=True)
     print(f"[{tag}] excess over FP64 (p50 p99 p99.9 max): gpu {_q(eg)}, FP32 oracle {_q(e32)}",
           flush=True)
     if gpu_first_f is not None:
@@ -103,7 +114,8 @@ def check(tag, model, consts, y, gpu_params, r32, r64, gpu_first_f=None, min_agr
         out["first_f_ok"] = float(np.mean(ok))
         assert out["first_f_ok"] >= 0.999, (tag, "first_f", out["first_f_ok"])
     assert agree >= bar, (tag, "final SSR vs FP32 oracle", agree, bar)
-    assert par_g >= par_32 - 0.02, (tag, "parameters vs FP64", par_g, par_32)
+    out["params_discordant"] = disc
+    assert par_g >= par_32 - par_slack, (tag, "parameters vs FP64", par_g, par_32, par_slack)
     slack = 3e-4 + 2.0 / eg.size           # two voxels of a small sample
     for t in TAILS:
         mg, m32 = float(np.mean(eg > t)), float(np.mean(e32 > t))
