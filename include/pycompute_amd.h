@@ -285,6 +285,18 @@ int pca_mlp_mse_grad(const float* Y, const float* T, float* dY, float* loss, int
 /* out[n] += sum_m X[m*ld + n] (bias gradients). */
 int pca_mlp_colsum(const float* X, int64_t ld, int32_t M, int32_t N, float* out, void* stream);
 
+/* The same without atomics, so results repeat bit for bit: C = op(A) . op(B)
+ * (plain store), the K splits' partial products written to workspace (at
+ * least splits * M * N floats) and summed in split order; out[n] += the column
+ * sums of 64-row chunks (at most 1024 chunks; workspace at least chunks * N
+ * floats), summed in chunk order. */
+int pca_mlp_gemm_ordered(const float* A, int64_t lda, int32_t transA, const float* B, int64_t ldb,
+                         int32_t transB, float* C, int64_t ldc, int32_t M, int32_t N, int32_t K,
+                         int32_t splits, float* workspace, int64_t workspace_floats,
+                         void* stream);
+int pca_mlp_colsum_ordered(const float* X, int64_t ld, int32_t M, int32_t N, float* out,
+                           float* workspace, int64_t workspace_floats, void* stream);
+
 /* One Adam step (bias-corrected, step >= 1) on a flat fp32 parameter vector. */
 int pca_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
                  float b2, float eps, int32_t step, float weight_decay, void* stream);

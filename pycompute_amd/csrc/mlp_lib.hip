@@ -150,6 +150,7 @@ struct GemmArgs {
   const float* A;
   const float* B;
   float* C;
+  float* ws;          // split-K partials (gridDim.z, M, N), summed in order after; null: atomics
   const float* bias;  // EPI_BIAS*: (N)
   float* Z;           // EPI_BIAS_GELU: pre-activation out (M, ldc) or null
   const float* Zin;   // EPI_GELU_GRAD: pre-activation in (M, ldc)
@@ -203,7 +204,10 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
         float v = acc[i][j][q];
         const long long o = (long long)m * g.ldc + n;
         if (split) {
-          atomicAdd(&g.C[o], v);
+          if (g.ws)
+            g.ws[((long long)blockIdx.z * g.M + m) * g.N + n] = v;
+          else
+            atomicAdd(&g.C[o], v);
           continue;
         }
         if (g.epi == EPI_BIAS_GELU) {
@@ -241,7 +245,7 @@ __global__ void k_mse_grad(const float* Y, const float* T, float* dY, float* los
 // and block adds them to out. Every thread works whatever N is (4-wide
 // output layers included).
 __global__ void __launch_bounds__(256) k_colsum(const float* X, long long ld, int M, int N,
-                                               float* out) {
+                                               float* out, float* ws) {
   __shared__ float red[256];
   const int c0 = blockIdx.x * 256;
   const int nc = min(256, N - c0);
@@ -257,7 +261,32 @@ __global__ void __launch_bounds__(256) k_colsum(const float* X, long long ld, in
   if (t < nc) {
     float sum = 0.f;
     for (int q = 0; q < groups; ++q) sum += red[q * nc + t];
-    atomicAdd(&out[c0 + t], sum);
+    if (ws)
+      ws[(long long)blockIdx.y * N + c0 + t] = sum;   // summed in order by k_sum_rows
+    else
+      atomicAdd(&out[c0 + t], sum);
+  }
+}
+
+// out[m * ldo + n] (+)= sum over z < Z of ws[(z M + m) N + n], z in order:
+// the fixed-order end of the split-K products and column sums.
+__global__ void k_sum_rows(const float* ws, int Z, int M, int N, float* out, long long ldo,
+                           int accumulate) {
+  const long long MN = (long long)M * N;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < MN;
+       i += (long long)gridDim.x * blockDim.x) {
+    // 8 loads in flight, added in z order (the same bits as one at a time)
+    float s = 0.f;
+    for (int z0 = 0; z0 < Z; z0 += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = z0 + u < Z ? ws[(z0 + u) * MN + i] : 0.f;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (z0 + u < Z) s += v[u];
+    }
+    const long long o = (i / N) * ldo + i % N;
+    out[o] = accumulate ? out[o] + s : s;
   }
 }
 
@@ -973,7 +1002,7 @@ extern "C" int pca_mlp_gemm(const float* A, int64_t lda, int32_t transA, const f
   if (epi == EPI_GELU_GRAD && !Zin) return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm: needs Zin");
   if (splits > 1 && epi != EPI_STORE)
     return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm: split-K only with the plain epilogue");
-  GemmArgs g{A, B, C, bias, Z, Zin, lda, ldb, ldc, M, N, K, transA, transB, epi, 0};
+  GemmArgs g{A, B, C, nullptr, bias, Z, Zin, lda, ldb, ldc, M, N, K, transA, transB, epi, 0};
   int kps = (K + splits - 1) / splits;
   kps = (kps + 15) / 16 * 16;
   g.k_per_split = kps;
@@ -997,7 +1026,44 @@ extern "C" int pca_mlp_colsum(const float* X, int64_t ld, int32_t M, int32_t N, 
   // chunks of 64 rows (at most 1024 chunks): a few dependent loads per thread
   dim3 grid((N + 255) / 256, std::min(1024, (M + 63) / 64));
   hipLaunchKernelGGL(k_colsum, grid, dim3(256), 0, (hipStream_t)stream, X, (long long)ld, M, N,
-                     out);
+                     out, nullptr);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_gemm_ordered(const float* A, int64_t lda, int32_t transA, const float* B,
+                                    int64_t ldb, int32_t transB, float* C, int64_t ldc, int32_t M,
+                                    int32_t N, int32_t K, int32_t splits, float* workspace,
+                                    int64_t workspace_floats, void* stream) {
+  if (!A || !B || !C || !workspace || M <= 0 || N <= 0 || K <= 0 || splits < 1)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm_ordered: bad argument");
+  int kps = (K + splits - 1) / splits;
+  kps = (kps + 15) / 16 * 16;
+  const int zs = (K + kps - 1) / kps;
+  if (workspace_floats < (int64_t)zs * M * N)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_gemm_ordered: workspace below splits * M * N floats");
+  GemmArgs g{A, B, C, workspace, nullptr, nullptr, nullptr, lda, ldb, ldc, M, N, K, transA,
+             transB, EPI_STORE, kps};
+  if (zs == 1) g.ws = nullptr;   // one split: stored directly
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_gemm, dim3((M + 63) / 64, (N + 63) / 64, zs), dim3(256), 0, s, g);
+  if (zs > 1)
+    hipLaunchKernelGGL(k_sum_rows, dim3(grid_1d((long long)M * N)), dim3(256), 0, s, workspace,
+                       zs, M, N, C, (long long)ldc, 0);
+  return check_launch();
+}
+
+extern "C" int pca_mlp_colsum_ordered(const float* X, int64_t ld, int32_t M, int32_t N, float* out,
+                                      float* workspace, int64_t workspace_floats, void* stream) {
+  if (!X || !out || !workspace || M <= 0 || N <= 0)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_colsum_ordered: bad argument");
+  const int ny = std::min(1024, (M + 63) / 64);
+  if (workspace_floats < (int64_t)ny * N)
+    return pca_fail(PCA_ERR_ARG, "pca_mlp_colsum_ordered: workspace below chunks * N floats");
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256, ny), dim3(256), 0, s, X, (long long)ld, M, N,
+                     out, workspace);
+  hipLaunchKernelGGL(k_sum_rows, dim3(grid_1d(N)), dim3(256), 0, s, workspace, ny, 1, N, out,
+                     (long long)N, 1);
   return check_launch();
 }
 

@@ -54,6 +54,9 @@ def _bind():
                                _P, _P, _P, _I32, _P]
     L.pca_mlp_mse_grad.argtypes = [_P, _P, _P, _P, _I64, _F, _P]
     L.pca_mlp_colsum.argtypes = [_P, _I64, _I32, _I32, _P, _P]
+    L.pca_mlp_gemm_ordered.argtypes = [_P, _I64, _I32, _P, _I64, _I32, _P, _I64, _I32, _I32, _I32,
+                                       _I32, _P, _I64, _P]
+    L.pca_mlp_colsum_ordered.argtypes = [_P, _I64, _I32, _I32, _P, _P, _I64, _P]
     L.pca_mlp_adam.argtypes = [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _F, _P]
     L.pca_mlp_fill.argtypes = [_P, _I64, _F, _P]
     L.pca_mlp_to_bf16.argtypes = [_P, _P, _I64, _P]
@@ -67,7 +70,8 @@ def _bind():
                                       _I32, _F, _P]
     for n in ("pca_mlp_gemm", "pca_mlp_mse_grad", "pca_mlp_colsum", "pca_mlp_adam",
               "pca_mlp_fill", "pca_mlp_to_bf16", "pca_mlp_to_f16", "pca_mlp_prepare",
-              "pca_mlp_infer", "pca_mlp_train_step", "pca_mlp_grad_reduce"):
+              "pca_mlp_infer", "pca_mlp_train_step", "pca_mlp_grad_reduce",
+              "pca_mlp_gemm_ordered", "pca_mlp_colsum_ordered"):
         getattr(L, n).restype = ctypes.c_int
     L._mlp_bound = True
     return L
@@ -230,6 +234,11 @@ class MLPInit:
                          "Y": torch.empty((B, O), **kw), "dY": torch.empty((B, O), **kw),
                          "dZ2": torch.empty((B, H), **kw), "dZ1": torch.empty((B, H), **kw),
                          "loss": torch.zeros(1, **kw)}
+            # partial products of the split weight gradients and the chunk
+            # column sums, summed in a fixed order (no atomics)
+            split = max(1, min(64, B // 256))
+            nws = max(split * H * max(H, self.INP), min(1024, (B + 63) // 64) * H)
+            self._buf["ws"] = torch.empty(nws, **kw)
         return self._buf
 
     def forward_backward(self, X, T, off: int, B: int, ldx: int):
@@ -250,19 +259,28 @@ class MLPInit:
         _lib.check(L.pca_mlp_mse_grad(b["Y"].data_ptr(), Tb.data_ptr(), b["dY"].data_ptr(),
                                       b["loss"].data_ptr(), B * O, 1.0 / (B * O), _stream()),
                    "pca_mlp_mse_grad")
-        # backward
+        # backward: weight gradients and bias sums without atomics, so a step
+        # repeats bit for bit
         _lib.check(L.pca_mlp_fill(self.g.data_ptr(), self.nparams, 0.0, _stream()), "fill")
-        gemm(b["dY"], O, 1, b["H2"], H, 0, g["w3"], H, O, H, B, splits=split)
-        _lib.check(L.pca_mlp_colsum(b["dY"].data_ptr(), O, B, O, g["b3"].data_ptr(), _stream()),
-                   "colsum")
+        ws, nws = b["ws"].data_ptr(), b["ws"].numel()
+
+        def wgrad(A, lda, Bm, ldb, transB, C, M, N):
+            _lib.check(L.pca_mlp_gemm_ordered(A.data_ptr(), lda, 1, Bm.data_ptr(), ldb, transB,
+                                              C.data_ptr(), N, M, N, B, split, ws, nws,
+                                              _stream()), "pca_mlp_gemm_ordered")
+
+        def bsum(X, N, out):
+            _lib.check(L.pca_mlp_colsum_ordered(X.data_ptr(), N, B, N, out.data_ptr(), ws, nws,
+                                                _stream()), "pca_mlp_colsum_ordered")
+
+        wgrad(b["dY"], O, b["H2"], H, 0, g["w3"], O, H)
+        bsum(b["dY"], O, g["b3"])
         gemm(b["dY"], O, 0, w["w3"], H, 0, b["dZ2"], H, B, H, O, EPI_GELU_GRAD, Zin=b["Z2"])
-        gemm(b["dZ2"], H, 1, b["H1"], H, 0, g["w2"], H, H, H, B, splits=split)
-        _lib.check(L.pca_mlp_colsum(b["dZ2"].data_ptr(), H, B, H, g["b2"].data_ptr(), _stream()),
-                   "colsum")
+        wgrad(b["dZ2"], H, b["H1"], H, 0, g["w2"], H, H)
+        bsum(b["dZ2"], H, g["b2"])
         gemm(b["dZ2"], H, 0, w["w2"], H, 0, b["dZ1"], H, B, H, H, EPI_GELU_GRAD, Zin=b["Z1"])
-        gemm(b["dZ1"], H, 1, Xb, ldx, 1, g["w1"], I, H, I, B, splits=split)
-        _lib.check(L.pca_mlp_colsum(b["dZ1"].data_ptr(), H, B, H, g["b1"].data_ptr(), _stream()),
-                   "colsum")
+        wgrad(b["dZ1"], H, Xb, ldx, 1, g["w1"], H, I)
+        bsum(b["dZ1"], H, g["b1"])
 
     # -- fused training step ------------------------------------------------------
     FUSED_ROWS = 64   # batch rows per workgroup of pca_mlp_train_step

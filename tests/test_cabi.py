@@ -111,6 +111,12 @@ def test_mlp_train_step_validation():
     assert L.pca_mlp_grad_reduce(1, 1, 128, np_, 1, None, 1, 1, 1, 1e-3, 0.9, 0.999, 1e-8, 0,
                                  0.0, None) == -1          # Adam needs step >= 1
     assert b"pca_mlp_grad_reduce" in L.pca_last_error()
+    # the ordered GEMM and column sums check their workspace size
+    assert L.pca_mlp_gemm_ordered(1, 64, 1, 1, 64, 0, 1, 64, 64, 64, 8192, 32, 1,
+                                  32 * 64 * 64 - 1, None) == -1
+    assert b"workspace" in L.pca_last_error()
+    assert L.pca_mlp_colsum_ordered(1, 64, 8192, 64, 1, 1, 128 * 64 - 1, None) == -1
+    assert b"workspace" in L.pca_last_error()
 
 
 def test_zero_voxel_calls_accept_empty_arrays():

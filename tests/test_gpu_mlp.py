@@ -75,6 +75,72 @@ def test_colsum_matches_float64(gpu, M, N, ld):
     assert err <= 1e-5 * max(1.0, float(np.sqrt(M))), err
 
 
+@pytest.mark.parametrize("transB,M,N,K,splits", [(0, 4, 64, 8192, 32), (0, 64, 64, 8192, 32),
+                                                  (1, 64, 16, 8192, 32), (0, 100, 36, 1000, 3),
+                                                  (1, 256, 256, 4096, 16), (0, 17, 5, 40, 1)])
+def test_gemm_ordered_matches_torch_and_repeats(gpu, transB, M, N, K, splits):
+    """pca_mlp_gemm_ordered (transA, the weight-gradient form): against torch
+    on bf16-rounded operands, into a C with a wider row stride, and the same
+    bits on a second call (split partials summed in order, no atomics)."""
+    from pycompute_amd import _lib
+    g = torch.Generator(device="cuda").manual_seed(M + N + K)
+    A = torch.randn((K, M), device="cuda", generator=g)
+    B = torch.randn((N, K) if transB else (K, N), device="cuda", generator=g)
+    ref = _bf(A).T @ (_bf(B).T if transB else _bf(B))
+    L = mlp._bind()
+    ws = torch.empty(splits * M * N, device="cuda")
+    outs = []
+    for _ in range(2):
+        C = torch.full((M, N + 3), 7.0, device="cuda")
+        _lib.check(L.pca_mlp_gemm_ordered(A.data_ptr(), M, 1, B.data_ptr(), B.shape[1], transB,
+                                          C.data_ptr(), N + 3, M, N, K, splits, ws.data_ptr(),
+                                          ws.numel(), torch.cuda.current_stream().cuda_stream),
+                   "gemm_ordered")
+        torch.cuda.synchronize()
+        assert torch.all(C[:, N:] == 7.0)                 # the pad columns untouched
+        outs.append(C[:, :N].clone())
+    err = (outs[0] - ref).abs().max().item() / ref.abs().max().item()
+    assert err < 1e-5, err
+    assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("M,N", [(8192, 4), (8192, 256), (100, 22)])
+def test_colsum_ordered_matches_float64_and_repeats(gpu, M, N):
+    from pycompute_amd import _lib
+    g = torch.Generator(device="cuda").manual_seed(M * N)
+    X = torch.randn((M, N), device="cuda", generator=g)
+    base = torch.randn(N, device="cuda", generator=g)
+    want = base.double() + X.double().sum(0)
+    L = mlp._bind()
+    ws = torch.empty(1024 * N, device="cuda")
+    outs = []
+    for _ in range(2):
+        out = base.clone()
+        _lib.check(L.pca_mlp_colsum_ordered(X.data_ptr(), N, M, N, out.data_ptr(), ws.data_ptr(),
+                                            ws.numel(), torch.cuda.current_stream().cuda_stream),
+                   "colsum_ordered")
+        torch.cuda.synchronize()
+        outs.append(out)
+    assert (outs[0].double() - want).abs().max().item() <= 1e-5 * max(1.0, float(np.sqrt(M)))
+    assert torch.equal(outs[0], outs[1])
+
+
+def test_gemm_path_training_repeats(gpu):
+    """Two trainings on the GEMM path (forced; the path of width-256 networks)
+    from the same initial weights and data end with the same weight bits."""
+    ws = []
+    for _ in range(2):
+        net = mlp.ivim_mlp(synth.B_C2, seed=11)
+        data, truth = net.synthesize(1 << 18, seed=5, snr=(20.0, 100.0),
+                                     lo=[0.8, 0.05, 5e-3, 0.5e-3], hi=[1.2, 0.4, 0.1, 3e-3])
+        net.fit_standardization(truth, data)
+        X, T = net.prepare(data, truth)
+        net.train(X, T, epochs=1, batch=8192, fused=False)
+        torch.cuda.synchronize()
+        ws.append(net.w.clone())
+    assert torch.equal(ws[0], ws[1])
+
+
 def _small_net(seed=0):
     net = mlp.ivim_mlp(synth.B_C2, seed=seed)
     return net
